@@ -1,0 +1,76 @@
+// Shared definitions for libhmx (MI355X / gfx950 Horn-Schunck hot path).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/hmx.h"
+
+namespace hmx {
+
+// Thread-local error text returned by hmx_last_error().
+void set_error(const std::string &msg);
+const std::string &get_error();
+
+// Status helpers: every entry point returns HMX_OK or one of the codes below
+// after recording a message.
+int fail(int code, const std::string &msg);
+
+#define HMX_HIP_TRY(expr)                                                       \
+    do {                                                                        \
+        hipError_t _e = (expr);                                                 \
+        if (_e != hipSuccess)                                                   \
+            return ::hmx::fail(HMX_ERR_HIP, std::string(#expr) + ": " +         \
+                                                hipGetErrorString(_e));         \
+    } while (0)
+
+#define HMX_LAUNCH_CHECK(what)                                                  \
+    do {                                                                        \
+        hipError_t _e = hipGetLastError();                                      \
+        if (_e != hipSuccess)                                                   \
+            return ::hmx::fail(HMX_ERR_HIP, std::string("launch ") + (what) +   \
+                                                ": " + hipGetErrorString(_e));  \
+    } while (0)
+
+#define HMX_REQUIRE(cond, msg)                                                  \
+    do {                                                                        \
+        if (!(cond)) return ::hmx::fail(HMX_ERR_INVALID, (msg));                \
+    } while (0)
+
+// Stencil taps passed by value as a kernel argument (uniform -> SGPRs).
+struct Taps {
+    int nd;
+    int rx, ry;  // max |offx|, max |offy|
+    float w[HMX_MAX_TAPS];
+    int ox[HMX_MAX_TAPS];
+    int oy[HMX_MAX_TAPS];
+};
+
+// Fill a Taps struct from host arrays; validates nd.
+int make_taps(Taps &t, int nd, const float *w, const int *ox, const int *oy);
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+__host__ __device__ inline int clampi(int v, int lo, int hi) {
+    return v < lo ? lo : (v > hi ? hi : v);
+}
+
+// blockIdx -> logical tile index so that the tiles one XCD receives (blocks
+// are dealt round-robin over the 8 XCDs) are spatially contiguous, which lets
+// neighbouring tiles share halo lines in that XCD's L2.
+__device__ inline int xcd_remap(int b, int n) {
+    constexpr int NX = 8;
+    int per = n / NX, rem = n % NX;
+    int x = b % NX, slot = b / NX;
+    return (x < rem) ? x * (per + 1) + slot : rem * (per + 1) + (x - rem) * per + slot;
+}
+
+// Device-side helpers used by several kernels.
+__global__ void fill_f32_kernel(float *p, float v, int64_t n);
+__global__ void fill_u32_kernel(unsigned *p, unsigned v, int64_t n);
+int launch_fill_f32(float *p, float v, int64_t n, hipStream_t s);
+int launch_fill_u32(unsigned *p, unsigned v, int64_t n, hipStream_t s);
+
+}  // namespace hmx

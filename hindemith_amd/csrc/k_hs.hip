@@ -1,0 +1,396 @@
+// a9: fused, temporally blocked Horn-Schunck Jacobi sweep for gfx950.
+//
+// One launch advances the flow field by k iterations.  A work-group owns a
+// TW x TH output tile.  It loads the tile grown by k pixels on each side
+// (clipped to the image) once: u and v interleaved as float2 in LDS, and Ix,
+// Iy, It plus den = alpha2 + Ix*Ix + Iy*Iy in registers of the thread that
+// owns each pixel.  It then runs k Jacobi steps entirely on chip; step j
+// updates the tile grown by k-j pixels, which is exactly the set whose
+// neighbours are still valid, so no halo exchange is needed inside a launch.
+// HBM traffic per launch is ~28 B per region pixel instead of 28 B per pixel
+// per iteration; redundant work is the (TW+2k)(TH+2k)/(TW*TH) - 1 overlap.
+//
+// Per-pixel arithmetic is the canonical program's, one rounding per op:
+//   ubar = Avg*u (fma chain in tap order), vbar = Avg*v,
+//   t = ((Ix*ubar + Iy*vbar) + It) / den, u' = ubar - Ix*t, v' = vbar - Iy*t
+// with den = (alpha2 + Ix*Ix) + Iy*Iy, so the result is bit-identical to
+// running the reference's per-op kernels with that rounding order.
+//
+// Replicated borders (Stencil2DMVM.cpp:42-45 clamp): a tile whose grown
+// region was clipped by the image uses clamped LDS coordinates; all others use
+// precomputed tap deltas.
+#include "hmx_common.h"
+
+#include <map>
+#include <mutex>
+#include <utility>
+#include <vector>
+
+#pragma clang fp contract(off)
+
+namespace hmx {
+
+struct HsArgs {
+    const float *Ix, *Iy, *It, *den;
+    const float *u_in, *v_in;
+    float *u_out, *v_out;
+    float alpha2;
+    int H, W, k, tiles_x, ntiles;
+};
+
+template <int NT, int TW, int TH, int KMAX>
+__global__ __launch_bounds__(NT) void hs_tb_kernel(HsArgs a, Taps taps) {
+    constexpr int RWM = TW + 2 * KMAX, RHM = TH + 2 * KMAX;
+    constexpr int NPX = RWM * RHM;
+    constexpr int P = (NPX + NT - 1) / NT;
+    __shared__ float2 uv[NPX];
+
+    const int tid = threadIdx.x;
+    const int t = xcd_remap(blockIdx.x, a.ntiles);
+    const int tyi = t / a.tiles_x, txi = t - tyi * a.tiles_x;
+    const int x0 = txi * TW, y0 = tyi * TH;
+    const int x1 = min(x0 + TW, a.W), y1 = min(y0 + TH, a.H);
+    const int k = a.k;
+    const int gx0 = max(x0 - k, 0), gy0 = max(y0 - k, 0);
+    const int gx1 = min(x1 + k, a.W), gy1 = min(y1 + k, a.H);
+    const int RW = gx1 - gx0, RH = gy1 - gy0, npx = RW * RH;
+    const bool clipped = (x0 - k < 0) || (y0 - k < 0) || (x1 + k > a.W) || (y1 + k > a.H);
+
+    float cx[P], cy[P], ct[P], cd[P];
+    int pk[P];  // (ry << 16) | rx of each owned region pixel
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        const int i = tid + j * NT;
+        pk[j] = -1;
+        cx[j] = cy[j] = ct[j] = 0.0f;
+        cd[j] = 1.0f;
+        if (i < npx) {
+            const int ry = i / RW, rx = i - ry * RW;
+            pk[j] = (ry << 16) | rx;
+            const int64_t g = (int64_t)(gy0 + ry) * a.W + gx0 + rx;
+            const float ix = a.Ix[g], iy = a.Iy[g];
+            cx[j] = ix;
+            cy[j] = iy;
+            ct[j] = a.It[g];
+            if (a.den) {
+                cd[j] = a.den[g];
+            } else {
+                const float m1 = ix * ix;
+                const float s1 = a.alpha2 + m1;
+                const float m2 = iy * iy;
+                cd[j] = s1 + m2;
+            }
+            uv[i] = make_float2(a.u_in[g], a.v_in[g]);
+        }
+    }
+    __syncthreads();
+
+    for (int it = 1; it <= k; ++it) {
+        const int e = k - it;
+        const int rx0 = max(x0 - e, 0) - gx0, ry0 = max(y0 - e, 0) - gy0;
+        const int rx1 = min(x1 + e, a.W) - gx0, ry1 = min(y1 + e, a.H) - gy0;
+        float nu[P], nv[P];
+        bool act[P];
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            const int rx = pk[j] & 0xffff, ry = pk[j] >> 16;
+            act[j] = pk[j] >= 0 && rx >= rx0 && rx < rx1 && ry >= ry0 && ry < ry1;
+            nu[j] = nv[j] = 0.0f;
+            if (act[j]) {
+                float ub = 0.0f, vb = 0.0f;
+                if (!clipped) {
+                    const int i = ry * RW + rx;
+                    for (int d = 0; d < taps.nd; ++d) {
+                        const float2 q = uv[i + taps.oy[d] * RW + taps.ox[d]];
+                        ub = fmaf(taps.w[d], q.x, ub);
+                        vb = fmaf(taps.w[d], q.y, vb);
+                    }
+                } else {
+                    const int gx = gx0 + rx, gy = gy0 + ry;
+                    for (int d = 0; d < taps.nd; ++d) {
+                        const int nx = clampi(gx + taps.ox[d], 0, a.W - 1) - gx0;
+                        const int ny = clampi(gy + taps.oy[d], 0, a.H - 1) - gy0;
+                        const float2 q = uv[ny * RW + nx];
+                        ub = fmaf(taps.w[d], q.x, ub);
+                        vb = fmaf(taps.w[d], q.y, vb);
+                    }
+                }
+                const float m1 = cx[j] * ub;
+                const float m2 = cy[j] * vb;
+                const float s1 = m1 + m2;
+                const float s2 = s1 + ct[j];
+                const float tt = s2 / cd[j];
+                const float p1 = cx[j] * tt;
+                const float p2 = cy[j] * tt;
+                nu[j] = ub - p1;
+                nv[j] = vb - p2;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < P; ++j)
+            if (act[j]) uv[(pk[j] >> 16) * RW + (pk[j] & 0xffff)] = make_float2(nu[j], nv[j]);
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        if (pk[j] < 0) continue;
+        const int rx = pk[j] & 0xffff, ry = pk[j] >> 16;
+        const int gx = gx0 + rx, gy = gy0 + ry;
+        if (gx >= x0 && gx < x1 && gy >= y0 && gy < y1) {
+            const int64_t g = (int64_t)gy * a.W + gx;
+            const float2 q = uv[ry * RW + rx];
+            a.u_out[g] = q.x;
+            a.v_out[g] = q.y;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Variant table.  (threads, tile, max iterations per launch); LDS use is
+// 8*(TW+2K)(TH+2K) bytes.
+struct HsVariant {
+    int nt, tw, th, kmax;
+    void (*kernel)(HsArgs, Taps);
+};
+
+#define HS_V(NT, TW, TH, K) {NT, TW, TH, K, hs_tb_kernel<NT, TW, TH, K>}
+static const HsVariant kVariants[] = {
+    HS_V(256, 64, 16, 1),     // 0: one iteration per launch (plain sweep)
+    HS_V(256, 64, 32, 4),     // 1
+    HS_V(512, 128, 32, 4),    // 2
+    HS_V(512, 64, 64, 8),     // 3
+    HS_V(1024, 128, 64, 8),   // 4: 92 KB LDS
+    HS_V(1024, 256, 32, 4),   // 5: 84 KB LDS
+    HS_V(256, 64, 32, 2),     // 6
+};
+#undef HS_V
+static constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+
+static std::mutex g_tune_mu;
+static std::map<std::pair<int, int>, std::pair<int, int>> g_tuned;  // (H,W) -> (variant,k)
+
+static void default_choice(int H, int W, int &variant, int &k) {
+    const int64_t n = (int64_t)H * W;
+    if (n >= (int64_t)1024 * 1024) {
+        variant = 4;
+        k = 8;
+    } else if (n >= 256 * 256) {
+        variant = 3;
+        k = 8;
+    } else {
+        variant = 1;
+        k = 4;
+    }
+}
+
+int launch_hs_tb(int variant, const float *Ix, const float *Iy, const float *It,
+                 const float *den, float alpha2, int H, int W, const Taps &taps, int k,
+                 const float *u_in, const float *v_in, float *u_out, float *v_out,
+                 hipStream_t s) {
+    const HsVariant &V = kVariants[variant];
+    HsArgs a;
+    a.Ix = Ix;
+    a.Iy = Iy;
+    a.It = It;
+    a.den = den;
+    a.u_in = u_in;
+    a.v_in = v_in;
+    a.u_out = u_out;
+    a.v_out = v_out;
+    a.alpha2 = alpha2;
+    a.H = H;
+    a.W = W;
+    a.k = k;
+    a.tiles_x = (W + V.tw - 1) / V.tw;
+    const int tiles_y = (H + V.th - 1) / V.th;
+    a.ntiles = a.tiles_x * tiles_y;
+    hipLaunchKernelGGL(V.kernel, dim3(a.ntiles), dim3(V.nt), 0, s, a, taps);
+    HMX_LAUNCH_CHECK("hs_tb");
+    return HMX_OK;
+}
+
+static int check_avg_taps(const Taps &t) {
+    if (t.rx > 1 || t.ry > 1)
+        return fail(HMX_ERR_UNSUPPORTED,
+                    "fused HS Jacobi needs an averaging stencil within radius 1");
+    return HMX_OK;
+}
+
+// iters Jacobi steps, ping-ponging between (u,v) and (wu,wv); the result ends
+// in (u,v).
+int hs_jacobi_run(const float *Ix, const float *Iy, const float *It, const float *den,
+                  float alpha2, int H, int W, const Taps &taps, int iters, float *u, float *v,
+                  float *wu, float *wv, int variant, int k, hipStream_t s) {
+    if (iters <= 0) return HMX_OK;
+    float *src_u = u, *src_v = v, *dst_u = wu, *dst_v = wv;
+    int done = 0;
+    while (done < iters) {
+        const int kk = (iters - done) < k ? (iters - done) : k;
+        int rc = launch_hs_tb(variant, Ix, Iy, It, den, alpha2, H, W, taps, kk, src_u, src_v,
+                              dst_u, dst_v, s);
+        if (rc) return rc;
+        std::swap(src_u, dst_u);
+        std::swap(src_v, dst_v);
+        done += kk;
+    }
+    if (src_u != u) {
+        HMX_HIP_TRY(hipMemcpyAsync(u, src_u, sizeof(float) * (size_t)H * W,
+                                   hipMemcpyDeviceToDevice, s));
+        HMX_HIP_TRY(hipMemcpyAsync(v, src_v, sizeof(float) * (size_t)H * W,
+                                   hipMemcpyDeviceToDevice, s));
+    }
+    return HMX_OK;
+}
+
+void hs_choice(int H, int W, int &variant, int &k) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find({H, W});
+    if (it != g_tuned.end()) {
+        variant = it->second.first;
+        k = it->second.second;
+    } else {
+        default_choice(H, W, variant, k);
+    }
+}
+
+static int validate_jacobi(const float *Ix, const float *Iy, const float *It, int H, int W,
+                           float *u, float *v, float *wu, float *wv) {
+    HMX_REQUIRE(Ix && Iy && It && u && v && wu && wv, "hs_jacobi: null array");
+    HMX_REQUIRE(H > 0 && W > 0, "hs_jacobi: H and W must be positive");
+    HMX_REQUIRE(H < 32768 && W < 32768, "hs_jacobi: H and W must be below 32768");
+    HMX_REQUIRE(u != wu && v != wv && u != v && wu != wv,
+                "hs_jacobi: u, v, work_u, work_v must be four distinct buffers");
+    return HMX_OK;
+}
+
+}  // namespace hmx
+
+using namespace hmx;
+
+extern "C" int hmx_hs_variant_count(void) { return kNumVariants; }
+
+extern "C" int hmx_hs_variant_info(int variant, int *tile_w, int *tile_h, int *threads,
+                                   int *kmax) {
+    HMX_REQUIRE(variant >= 0 && variant < kNumVariants, "unknown HS kernel variant");
+    const HsVariant &V = kVariants[variant];
+    if (tile_w) *tile_w = V.tw;
+    if (tile_h) *tile_h = V.th;
+    if (threads) *threads = V.nt;
+    if (kmax) *kmax = V.kmax;
+    return HMX_OK;
+}
+
+extern "C" int hmx_hs_jacobi_cfg(const float *Ix, const float *Iy, const float *It,
+                                 const float *den, float alpha2, int H, int W, int nda,
+                                 const float *wa, const int *oxa, const int *oya, int iters,
+                                 float *u, float *v, float *work_u, float *work_v, int variant,
+                                 int k, void *stream) {
+    int rc = validate_jacobi(Ix, Iy, It, H, W, u, v, work_u, work_v);
+    if (rc) return rc;
+    HMX_REQUIRE(iters >= 0, "hs_jacobi: iters must be >= 0");
+    HMX_REQUIRE(variant >= 0 && variant < kNumVariants, "hs_jacobi: unknown kernel variant");
+    HMX_REQUIRE(k >= 1 && k <= kVariants[variant].kmax,
+                "hs_jacobi: k must be in 1..kmax of the variant");
+    Taps t;
+    rc = make_taps(t, nda, wa, oxa, oya);
+    if (!rc) rc = check_avg_taps(t);
+    if (rc) return rc;
+    return hs_jacobi_run(Ix, Iy, It, den, alpha2, H, W, t, iters, u, v, work_u, work_v, variant,
+                         k, as_stream(stream));
+}
+
+extern "C" int hmx_hs_jacobi(const float *Ix, const float *Iy, const float *It,
+                             const float *den, float alpha2, int H, int W, int nda,
+                             const float *wa, const int *oxa, const int *oya, int iters,
+                             float *u, float *v, float *work_u, float *work_v, void *stream) {
+    int variant, k;
+    hs_choice(H, W, variant, k);
+    return hmx_hs_jacobi_cfg(Ix, Iy, It, den, alpha2, H, W, nda, wa, oxa, oya, iters, u, v,
+                             work_u, work_v, variant, k, stream);
+}
+
+extern "C" int hmx_hs_tune(const float *Ix, const float *Iy, const float *It, float alpha2,
+                           int H, int W, int nda, const float *wa, const int *oxa,
+                           const int *oya, float *u, float *v, float *work_u, float *work_v,
+                           void *stream, int *variant_out, int *k_out) {
+    int rc = validate_jacobi(Ix, Iy, It, H, W, u, v, work_u, work_v);
+    if (rc) return rc;
+    Taps t;
+    rc = make_taps(t, nda, wa, oxa, oya);
+    if (!rc) rc = check_avg_taps(t);
+    if (rc) return rc;
+    hipStream_t s = as_stream(stream);
+    hipEvent_t e0, e1;
+    HMX_HIP_TRY(hipEventCreate(&e0));
+    HMX_HIP_TRY(hipEventCreate(&e1));
+    const int probe_iters = 48;  // divisible by every k tried
+    float best = 1e30f;
+    int best_v = 0, best_k = 1;
+    for (int vi = 0; vi < kNumVariants; ++vi) {
+        for (int k = 1; k <= kVariants[vi].kmax; k *= 2) {
+            // warm-up, then timed
+            rc = hs_jacobi_run(Ix, Iy, It, nullptr, alpha2, H, W, t, probe_iters, u, v, work_u,
+                               work_v, vi, k, s);
+            if (rc) break;
+            HMX_HIP_TRY(hipEventRecord(e0, s));
+            rc = hs_jacobi_run(Ix, Iy, It, nullptr, alpha2, H, W, t, probe_iters, u, v, work_u,
+                               work_v, vi, k, s);
+            if (rc) break;
+            HMX_HIP_TRY(hipEventRecord(e1, s));
+            HMX_HIP_TRY(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            HMX_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) {
+                best = ms;
+                best_v = vi;
+                best_k = k;
+            }
+        }
+        if (rc) break;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) return rc;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned[{H, W}] = {best_v, best_k};
+    }
+    if (variant_out) *variant_out = best_v;
+    if (k_out) *k_out = best_k;
+    return HMX_OK;
+}
+
+namespace hmx {
+int launch_hs_setup(const float *I1, const float *I2, int H, int W, const Taps &dx,
+                    const Taps &dy, const Taps &dt, float *Ix, float *Iy, float *It,
+                    hipStream_t s);
+}
+
+extern "C" int hmx_hs_solve(const float *I1, const float *I2, int H, int W, float alpha2,
+                            int ndx, const float *wx, const int *oxx, const int *oyx, int ndy,
+                            const float *wy, const int *oxy, const int *oyy, int ndt,
+                            const float *wt, const int *oxt, const int *oyt, int nda,
+                            const float *wa, const int *oxa, const int *oya, int iters,
+                            float *u, float *v, float *workspace, void *stream) {
+    HMX_REQUIRE(I1 && I2 && u && v && workspace, "hmx_hs_solve: null array");
+    HMX_REQUIRE(H > 0 && W > 0, "hmx_hs_solve: H and W must be positive");
+    Taps dx, dy, dt, av;
+    int rc = make_taps(dx, ndx, wx, oxx, oyx);
+    if (!rc) rc = make_taps(dy, ndy, wy, oxy, oyy);
+    if (!rc) rc = make_taps(dt, ndt, wt, oxt, oyt);
+    if (!rc) rc = make_taps(av, nda, wa, oxa, oya);
+    if (!rc) rc = check_avg_taps(av);
+    if (rc) return rc;
+    const size_t n = (size_t)H * W;
+    float *Ix = workspace, *Iy = workspace + n, *It = workspace + 2 * n;
+    float *wu = workspace + 3 * n, *wv = workspace + 4 * n;
+    hipStream_t s = as_stream(stream);
+    rc = launch_hs_setup(I1, I2, H, W, dx, dy, dt, Ix, Iy, It, s);
+    if (rc) return rc;
+    int variant, k;
+    hs_choice(H, W, variant, k);
+    return hs_jacobi_run(Ix, Iy, It, nullptr, alpha2, H, W, av, iters, u, v, wu, wv, variant, k,
+                         s);
+}

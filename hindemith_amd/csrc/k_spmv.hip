@@ -1,0 +1,138 @@
+// a3 SpmvCSR for gfx950.
+//
+// The reference runs one work-item per row (SpmvCSR.cpp:32-38), which leaves
+// the data/indices reads uncoalesced and serialises long rows.  Here a row is
+// owned by a sub-wave of L lanes (L = 1..64 dividing the 64-wide wavefront):
+// the lanes stride the row's entries together, so a sub-wave reads L
+// consecutive (value, index) pairs per step, and the partial sums are folded
+// with cross-lane shuffles inside the sub-wave.  For irregular row lengths the
+// rows are binned by length and each bin gets its own L.
+#include "hmx_common.h"
+
+#include <vector>
+
+#pragma clang fp contract(off)
+
+namespace hmx {
+
+template <int L>
+__device__ inline float subwave_sum(float v) {
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, L);
+    return v;
+}
+
+// rows: optional row list (binned launch); nrows rows handled.
+template <int L>
+__global__ __launch_bounds__(256) void csr_vector_kernel(int nrows, const int *__restrict__ rows,
+                                                         const int *__restrict__ indptr,
+                                                         const int *__restrict__ indices,
+                                                         const float *__restrict__ data,
+                                                         const float *__restrict__ x,
+                                                         float *__restrict__ y) {
+    const int lane = threadIdx.x & (L - 1);
+    const int64_t sub = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / L;
+    const int64_t nsub = ((int64_t)gridDim.x * blockDim.x) / L;
+    // Whole sub-waves iterate together, so the shuffles below always see
+    // their full sub-wave active.
+    for (int64_t s = sub; s < nrows; s += nsub) {
+        const int row = rows ? rows[s] : (int)s;
+        const int beg = indptr[row], end = indptr[row + 1];
+        float acc = 0.0f;
+        for (int j = beg + lane; j < end; j += L) acc = fmaf(data[j], x[indices[j]], acc);
+        acc = subwave_sum<L>(acc);
+        if (lane == 0) y[row] = acc;
+    }
+}
+
+static int launch_csr_L(int L, int nrows, const int *rows, const int *indptr,
+                        const int *indices, const float *data, const float *x, float *y,
+                        hipStream_t s) {
+    if (nrows <= 0) return HMX_OK;
+    int64_t threads = (int64_t)nrows * L;
+    int64_t blocks = (threads + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    dim3 g((unsigned)blocks), b(256);
+    switch (L) {
+        case 1: hipLaunchKernelGGL(csr_vector_kernel<1>, g, b, 0, s, nrows, rows, indptr, indices, data, x, y); break;
+        case 2: hipLaunchKernelGGL(csr_vector_kernel<2>, g, b, 0, s, nrows, rows, indptr, indices, data, x, y); break;
+        case 4: hipLaunchKernelGGL(csr_vector_kernel<4>, g, b, 0, s, nrows, rows, indptr, indices, data, x, y); break;
+        case 8: hipLaunchKernelGGL(csr_vector_kernel<8>, g, b, 0, s, nrows, rows, indptr, indices, data, x, y); break;
+        case 16: hipLaunchKernelGGL(csr_vector_kernel<16>, g, b, 0, s, nrows, rows, indptr, indices, data, x, y); break;
+        case 32: hipLaunchKernelGGL(csr_vector_kernel<32>, g, b, 0, s, nrows, rows, indptr, indices, data, x, y); break;
+        case 64: hipLaunchKernelGGL(csr_vector_kernel<64>, g, b, 0, s, nrows, rows, indptr, indices, data, x, y); break;
+        default: return fail(HMX_ERR_INVALID, "csr: lanes_per_row must be a power of two <= 64");
+    }
+    HMX_LAUNCH_CHECK("csr_vector");
+    return HMX_OK;
+}
+
+int csr_auto_lanes(int M, int64_t nnz) {
+    const double mean = M > 0 ? (double)nnz / M : 0.0;
+    int L = 1;
+    while (L < 64 && L < mean) L *= 2;
+    return L;
+}
+
+int launch_csr(int M, int64_t nnz, const int *indptr, const int *indices, const float *data,
+               const float *x, float *y, int L, hipStream_t s) {
+    if (L <= 0) L = csr_auto_lanes(M, nnz);
+    return launch_csr_L(L, M, nullptr, indptr, indices, data, x, y, s);
+}
+
+// Length classes of the binned launch and the sub-wave width used for each.
+static const int kBinMax[4] = {4, 16, 64, 1 << 30};
+static const int kBinLanes[4] = {4, 16, 64, 64};
+
+}  // namespace hmx
+
+using namespace hmx;
+
+extern "C" int hmx_csr_spmv(int M, int N, int64_t nnz, const int *indptr, const int *indices,
+                            const float *data, const float *x, float *y, int lanes_per_row,
+                            void *stream) {
+    HMX_REQUIRE(indptr && x && y, "hmx_csr_spmv: null array");
+    HMX_REQUIRE(M >= 0 && N >= 0 && nnz >= 0, "hmx_csr_spmv: negative size");
+    HMX_REQUIRE(nnz < ((int64_t)1 << 31), "hmx_csr_spmv: int32 indices limit nnz to 2^31-1");
+    HMX_REQUIRE(nnz == 0 || (indices && data), "hmx_csr_spmv: null indices/data");
+    HMX_REQUIRE((const void *)x != (const void *)y, "hmx_csr_spmv: x and y must be distinct");
+    return launch_csr(M, nnz, indptr, indices, data, x, y, lanes_per_row, as_stream(stream));
+}
+
+extern "C" int hmx_csr_bin_rows(int M, const int *indptr_host, int *bins_host, int *bin_counts) {
+    HMX_REQUIRE(M >= 0 && indptr_host && bins_host && bin_counts, "hmx_csr_bin_rows: bad args");
+    int cnt[4] = {0, 0, 0, 0};
+    for (int i = 0; i < M; ++i) {
+        const int len = indptr_host[i + 1] - indptr_host[i];
+        int b = 0;
+        while (len > kBinMax[b]) ++b;
+        ++cnt[b];
+    }
+    int off[4] = {0, cnt[0], cnt[0] + cnt[1], cnt[0] + cnt[1] + cnt[2]};
+    for (int i = 0; i < M; ++i) {
+        const int len = indptr_host[i + 1] - indptr_host[i];
+        int b = 0;
+        while (len > kBinMax[b]) ++b;
+        bins_host[off[b]++] = i;
+    }
+    for (int b = 0; b < 4; ++b) bin_counts[b] = cnt[b];
+    return HMX_OK;
+}
+
+extern "C" int hmx_csr_spmv_binned(int M, const int *indptr, const int *indices,
+                                   const float *data, const float *x, float *y, const int *bins,
+                                   const int *bin_counts, void *stream) {
+    HMX_REQUIRE(indptr && indices && data && x && y && bins && bin_counts,
+                "hmx_csr_spmv_binned: null argument");
+    int total = 0, off = 0;
+    for (int b = 0; b < 4; ++b) total += bin_counts[b];
+    HMX_REQUIRE(total == M, "hmx_csr_spmv_binned: bin counts do not add up to M");
+    hipStream_t s = as_stream(stream);
+    for (int b = 0; b < 4; ++b) {
+        int rc = launch_csr_L(kBinLanes[b], bin_counts[b], bins + off, indptr, indices, data, x,
+                              y, s);
+        if (rc) return rc;
+        off += bin_counts[b];
+    }
+    return HMX_OK;
+}

@@ -1,0 +1,203 @@
+// a1 Stencil2DMVM, a2 SpmvDIA2D and a10 HS setup kernels for gfx950.
+//
+// Arithmetic follows the reference's generated OpenCL: the accumulation
+// "y += D * x" of Stencil2DMVM.cpp:47 / SpmvDIA2D.cpp:43 is one fused
+// multiply-add per tap, in tap order, starting from y = 0.0; every other
+// binary op is rounded on its own (this file is compiled with contraction off).
+#include "hmx_common.h"
+
+#pragma clang fp contract(off)
+
+namespace hmx {
+
+// ---------------------------------------------------------------------------
+// a1: LDS-staged clamped stencil.  A 256-thread work-group (64 x 4) owns a
+// 64 x 16 output tile; it stages the (16+2R) x (64+2R) input window with
+// clamped coordinates, so every tap of every output pixel is an LDS read and
+// the border rule of Stencil2DMVM.cpp:40-46 (clamp each coordinate) holds by
+// construction: window cell (rx, ry) holds x[clamp(x0-R+rx), clamp(y0-R+ry)].
+constexpr int ST_TW = 64, ST_TH = 16;
+
+template <int R>
+__global__ __launch_bounds__(256) void stencil2d_lds_kernel(const float *__restrict__ x,
+                                                            float *__restrict__ y, int H, int W,
+                                                            Taps taps) {
+    constexpr int LW = ST_TW + 2 * R, LH = ST_TH + 2 * R;
+    __shared__ float tile[LH * LW];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int x0 = blockIdx.x * ST_TW, y0 = blockIdx.y * ST_TH;
+    for (int i = threadIdx.x; i < LH * LW; i += 256) {
+        int ry = i / LW, rx = i - ry * LW;
+        int gx = clampi(x0 - R + rx, 0, W - 1), gy = clampi(y0 - R + ry, 0, H - 1);
+        tile[i] = x[(int64_t)gy * W + gx];
+    }
+    __syncthreads();
+    const int gx = x0 + tx;
+    if (gx >= W) return;
+    for (int r = ty; r < ST_TH; r += 4) {
+        const int gy = y0 + r;
+        if (gy >= H) break;
+        const float *base = tile + (r + R) * LW + tx + R;
+        float acc = 0.0f;
+        for (int d = 0; d < taps.nd; ++d)
+            acc = fmaf(taps.w[d], base[taps.oy[d] * LW + taps.ox[d]], acc);
+        y[(int64_t)gy * W + gx] = acc;
+    }
+}
+
+// Wide stencils (radius > 4): taps straight from global memory (L1/L2 serve
+// the reuse), still clamped per coordinate.
+__global__ __launch_bounds__(256) void stencil2d_direct_kernel(const float *__restrict__ x,
+                                                               float *__restrict__ y, int H,
+                                                               int W, Taps taps) {
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= W) return;
+    for (int r = blockIdx.y; r < H; r += gridDim.y) {
+        float acc = 0.0f;
+        for (int d = 0; d < taps.nd; ++d) {
+            int xx = clampi(c + taps.ox[d], 0, W - 1), yy = clampi(r + taps.oy[d], 0, H - 1);
+            acc = fmaf(taps.w[d], x[(int64_t)yy * W + xx], acc);
+        }
+        y[(int64_t)r * W + c] = acc;
+    }
+}
+
+int launch_stencil2d(const float *x, float *y, int H, int W, const Taps &t, hipStream_t s) {
+    const int R = t.rx > t.ry ? t.rx : t.ry;
+    if (R <= 4) {
+        dim3 grid((W + ST_TW - 1) / ST_TW, (H + ST_TH - 1) / ST_TH);
+        if (R <= 1)
+            hipLaunchKernelGGL(stencil2d_lds_kernel<1>, grid, dim3(256), 0, s, x, y, H, W, t);
+        else if (R <= 2)
+            hipLaunchKernelGGL(stencil2d_lds_kernel<2>, grid, dim3(256), 0, s, x, y, H, W, t);
+        else
+            hipLaunchKernelGGL(stencil2d_lds_kernel<4>, grid, dim3(256), 0, s, x, y, H, W, t);
+    } else {
+        dim3 grid((W + 255) / 256, H < 65535 ? H : 65535);
+        hipLaunchKernelGGL(stencil2d_direct_kernel, grid, dim3(256), 0, s, x, y, H, W, t);
+    }
+    HMX_LAUNCH_CHECK("stencil2d");
+    return HMX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// a2: 2D DIA SpMV.  y[p] = sum_d A[d][p+o_d] * x[p+o_d] over in-range p+o_d
+// (SpmvDIA2D.cpp:41-43; coefficient index SpMat2D.cpp:214).  Each coefficient
+// plane element is read exactly once over the whole launch, so the planes are
+// streamed straight from HBM; the nd shifted reads of x hit in L1/L2.
+__global__ __launch_bounds__(256) void dia2d_kernel(const float *__restrict__ A,
+                                                    const float *__restrict__ x,
+                                                    float *__restrict__ y, int H, int W,
+                                                    Taps taps) {
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= W) return;
+    const int64_t HW = (int64_t)H * W;
+    for (int r = blockIdx.y; r < H; r += gridDim.y) {
+        const int64_t p = (int64_t)r * W + c;
+        float acc = 0.0f;
+        for (int d = 0; d < taps.nd; ++d) {
+            const int xx = c + taps.ox[d], yy = r + taps.oy[d];
+            if (xx >= 0 && xx < W && yy >= 0 && yy < H) {
+                const int64_t q = (int64_t)yy * W + xx;
+                acc = fmaf(A[d * HW + q], x[q], acc);
+            }
+        }
+        y[p] = acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// a10: the canonical derivative cube in one pass (20 B/px: read I1, I2;
+// write Ix, Iy, It).  Same per-op rounding as six Stencil2DMVM launches and
+// three Array2DOps.
+__device__ inline float tap_apply(const float *__restrict__ x, int H, int W, int r, int c,
+                                  const Taps &t) {
+    float acc = 0.0f;
+    for (int d = 0; d < t.nd; ++d) {
+        int xx = clampi(c + t.ox[d], 0, W - 1), yy = clampi(r + t.oy[d], 0, H - 1);
+        acc = fmaf(t.w[d], x[(int64_t)yy * W + xx], acc);
+    }
+    return acc;
+}
+
+__global__ __launch_bounds__(256) void hs_setup_kernel(const float *__restrict__ I1,
+                                                       const float *__restrict__ I2, int H,
+                                                       int W, Taps dx, Taps dy, Taps dt,
+                                                       float *__restrict__ Ix,
+                                                       float *__restrict__ Iy,
+                                                       float *__restrict__ It) {
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= W) return;
+    for (int r = blockIdx.y; r < H; r += gridDim.y) {
+        const int64_t p = (int64_t)r * W + c;
+        float a1 = tap_apply(I1, H, W, r, c, dx);
+        float a2 = tap_apply(I2, H, W, r, c, dx);
+        Ix[p] = a1 + a2;
+        float b1 = tap_apply(I1, H, W, r, c, dy);
+        float b2 = tap_apply(I2, H, W, r, c, dy);
+        Iy[p] = b1 + b2;
+        float t2 = tap_apply(I2, H, W, r, c, dt);
+        float t1 = tap_apply(I1, H, W, r, c, dt);
+        It[p] = t2 - t1;
+    }
+}
+
+int launch_hs_setup(const float *I1, const float *I2, int H, int W, const Taps &dx,
+                    const Taps &dy, const Taps &dt, float *Ix, float *Iy, float *It,
+                    hipStream_t s) {
+    dim3 grid((W + 255) / 256, H < 65535 ? H : 65535);
+    hipLaunchKernelGGL(hs_setup_kernel, grid, dim3(256), 0, s, I1, I2, H, W, dx, dy, dt, Ix, Iy,
+                       It);
+    HMX_LAUNCH_CHECK("hs_setup");
+    return HMX_OK;
+}
+
+int launch_dia2d(const float *A, const float *x, float *y, int H, int W, const Taps &t,
+                 hipStream_t s) {
+    dim3 grid((W + 255) / 256, H < 65535 ? H : 65535);
+    hipLaunchKernelGGL(dia2d_kernel, grid, dim3(256), 0, s, A, x, y, H, W, t);
+    HMX_LAUNCH_CHECK("dia2d");
+    return HMX_OK;
+}
+
+}  // namespace hmx
+
+using namespace hmx;
+
+extern "C" int hmx_stencil2d(const float *x, float *y, int H, int W, int nd, const float *w,
+                             const int *offx, const int *offy, void *stream) {
+    HMX_REQUIRE(x && y, "hmx_stencil2d: null array");
+    HMX_REQUIRE(x != y, "hmx_stencil2d: x and y must be distinct buffers");
+    HMX_REQUIRE(H > 0 && W > 0, "hmx_stencil2d: H and W must be positive");
+    Taps t;
+    int rc = make_taps(t, nd, w, offx, offy);
+    if (rc) return rc;
+    return launch_stencil2d(x, y, H, W, t, as_stream(stream));
+}
+
+extern "C" int hmx_dia2d_spmv(const float *A, int nd, const int *offx, const int *offy,
+                              const float *x, float *y, int H, int W, void *stream) {
+    HMX_REQUIRE(A && x && y, "hmx_dia2d_spmv: null array");
+    HMX_REQUIRE(x != y, "hmx_dia2d_spmv: x and y must be distinct buffers");
+    HMX_REQUIRE(H > 0 && W > 0, "hmx_dia2d_spmv: H and W must be positive");
+    static const float unused_w[HMX_MAX_TAPS] = {0};  // DIA weights live in A
+    Taps t;
+    int rc = make_taps(t, nd, unused_w, offx, offy);
+    if (rc) return rc;
+    return launch_dia2d(A, x, y, H, W, t, as_stream(stream));
+}
+
+extern "C" int hmx_hs_setup(const float *I1, const float *I2, int H, int W, int ndx,
+                            const float *wx, const int *oxx, const int *oyx, int ndy,
+                            const float *wy, const int *oxy, const int *oyy, int ndt,
+                            const float *wt, const int *oxt, const int *oyt, float *Ix,
+                            float *Iy, float *It, void *stream) {
+    HMX_REQUIRE(I1 && I2 && Ix && Iy && It, "hmx_hs_setup: null array");
+    HMX_REQUIRE(H > 0 && W > 0, "hmx_hs_setup: H and W must be positive");
+    Taps dx, dy, dt;
+    int rc = make_taps(dx, ndx, wx, oxx, oyx);
+    if (!rc) rc = make_taps(dy, ndy, wy, oxy, oyy);
+    if (!rc) rc = make_taps(dt, ndt, wt, oxt, oyt);
+    if (rc) return rc;
+    return launch_hs_setup(I1, I2, H, W, dx, dy, dt, Ix, Iy, It, as_stream(stream));
+}

@@ -1,0 +1,219 @@
+"""Device-tensor front doors to the hot-path kernels of libhmx.
+
+Each function takes torch tensors that already live on the GPU (fp32,
+contiguous), launches on torch's current stream and returns device tensors.
+Stencils are ``(weights, offx, offy)`` triples of host arrays, the same data a
+``hlib.Stencil`` carries.  These are what the autograd Function, the batched
+solver and the benchmark call; the ``@hm`` front end goes through the record
+API instead.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from ._lib import HindemithError, check, lib
+
+__all__ = [
+    "taps_of", "stencil2d", "dia2d_spmv", "csr_spmv", "CSRBins", "csr_spmv_binned",
+    "sum2d", "hs_setup", "hs_jacobi", "hs_solve", "hs_tune", "hs_variants", "stream_ptr",
+]
+
+
+def stream_ptr():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _dev(t, name, dtype=torch.float32):
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise HindemithError(f"{name} must be a CUDA/HIP tensor (there is no CPU path)")
+    if t.dtype != dtype:
+        raise HindemithError(f"{name} must be {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise HindemithError(f"{name} must be contiguous")
+    return t
+
+
+class _Taps:
+    """Host-side tap arrays kept alive for the duration of a call."""
+
+    def __init__(self, st):
+        if hasattr(st, "data") and hasattr(st, "offx"):
+            w, ox, oy = st.data, st.offx, st.offy
+        else:
+            w, ox, oy = st
+        self.w = np.ascontiguousarray(w, dtype=np.float32)
+        self.ox = np.ascontiguousarray(ox, dtype=np.int32)
+        self.oy = np.ascontiguousarray(oy, dtype=np.int32)
+        if not (len(self.w) == len(self.ox) == len(self.oy)):
+            raise HindemithError("stencil weights and offsets differ in length")
+
+    def args(self):
+        return [len(self.w), self.w.ctypes.data_as(ctypes.c_void_p),
+                self.ox.ctypes.data_as(ctypes.c_void_p),
+                self.oy.ctypes.data_as(ctypes.c_void_p)]
+
+
+def taps_of(st):
+    return _Taps(st)
+
+
+def stencil2d(x, st, out=None):
+    """a1: y = S * x (clamped borders)."""
+    _dev(x, "x")
+    H, W = x.shape
+    y = torch.empty_like(x) if out is None else _dev(out, "out")
+    t = _Taps(st)
+    check(lib().hmx_stencil2d(_ptr(x), _ptr(y), H, W, *t.args(), stream_ptr()), "stencil2d")
+    return y
+
+
+def dia2d_spmv(A, offx, offy, x, out=None):
+    """a2: y = A * x for a 2D DIA matrix A (nd x H x W planes)."""
+    _dev(A, "A")
+    _dev(x, "x")
+    nd, H, W = A.shape
+    if tuple(x.shape) != (H, W):
+        raise HindemithError("dia2d_spmv: x shape does not match the matrix planes")
+    ox = np.ascontiguousarray(offx, np.int32)
+    oy = np.ascontiguousarray(offy, np.int32)
+    y = torch.empty_like(x) if out is None else _dev(out, "out")
+    check(lib().hmx_dia2d_spmv(_ptr(A), nd, ox.ctypes.data_as(ctypes.c_void_p),
+                               oy.ctypes.data_as(ctypes.c_void_p), _ptr(x), _ptr(y), H, W,
+                               stream_ptr()), "dia2d_spmv")
+    return y
+
+
+def csr_spmv(indptr, indices, data, x, n_cols=None, out=None, lanes_per_row=0):
+    """a3: y = A * x, CSR with int32 indices (device tensors)."""
+    _dev(indptr, "indptr", torch.int32)
+    _dev(indices, "indices", torch.int32)
+    _dev(data, "data")
+    _dev(x, "x")
+    M = indptr.numel() - 1
+    y = torch.empty(M, device=x.device, dtype=torch.float32) if out is None else out
+    check(lib().hmx_csr_spmv(M, x.numel() if n_cols is None else n_cols, data.numel(),
+                             _ptr(indptr), _ptr(indices), _ptr(data), _ptr(x), _ptr(y),
+                             int(lanes_per_row), stream_ptr()), "csr_spmv")
+    return y
+
+
+class CSRBins:
+    """Row binning of a CSR matrix by row length (built once per matrix)."""
+
+    def __init__(self, indptr_host, device):
+        ip = np.ascontiguousarray(indptr_host, np.int32)
+        M = len(ip) - 1
+        bins = np.empty(max(M, 1), np.int32)
+        counts = np.zeros(4, np.int32)
+        check(lib().hmx_csr_bin_rows(M, ip.ctypes.data_as(ctypes.c_void_p),
+                                     bins.ctypes.data_as(ctypes.c_void_p),
+                                     counts.ctypes.data_as(ctypes.c_void_p)), "csr_bin_rows")
+        self.M = M
+        self.counts = counts
+        self.bins = torch.from_numpy(bins[:max(M, 1)]).to(device)
+
+
+def csr_spmv_binned(indptr, indices, data, x, bins, out=None):
+    M = indptr.numel() - 1
+    y = torch.empty(M, device=x.device, dtype=torch.float32) if out is None else out
+    check(lib().hmx_csr_spmv_binned(M, _ptr(indptr), _ptr(indices), _ptr(data), _ptr(x),
+                                    _ptr(y), _ptr(bins.bins),
+                                    bins.counts.ctypes.data_as(ctypes.c_void_p), stream_ptr()),
+          "csr_spmv_binned")
+    return y
+
+
+_SUM_PARTIALS = 1024
+
+
+def sum2d(x, partials=None, out=None):
+    """a7: deterministic sum of all elements -> 1-element fp32 device tensor."""
+    _dev(x, "x")
+    if partials is None:
+        partials = torch.empty(_SUM_PARTIALS, device=x.device, dtype=torch.float64)
+    y = torch.empty(1, device=x.device, dtype=torch.float32) if out is None else out
+    check(lib().hmx_sum2d(_ptr(x), x.numel(), _ptr(partials), _ptr(y), stream_ptr()), "sum2d")
+    return y
+
+
+def hs_setup(I1, I2, Dx, Dy, Dt, out=None):
+    """a10: (Ix, Iy, It) of the canonical program."""
+    _dev(I1, "I1")
+    _dev(I2, "I2")
+    H, W = I1.shape
+    if out is None:
+        out = torch.empty((3, H, W), device=I1.device, dtype=torch.float32)
+    Ix, Iy, It = out[0], out[1], out[2]
+    tx, ty, tt = _Taps(Dx), _Taps(Dy), _Taps(Dt)
+    check(lib().hmx_hs_setup(_ptr(I1), _ptr(I2), H, W, *tx.args(), *ty.args(), *tt.args(),
+                             _ptr(Ix), _ptr(Iy), _ptr(It), stream_ptr()), "hs_setup")
+    return Ix, Iy, It
+
+
+def hs_jacobi(Ix, Iy, It, alpha2, Avg, iters, u, v, work=None, den=None, variant=None, k=None):
+    """a9: ``iters`` fused Jacobi iterations, (u, v) updated in place."""
+    for n, t in (("Ix", Ix), ("Iy", Iy), ("It", It), ("u", u), ("v", v)):
+        _dev(t, n)
+    H, W = Ix.shape
+    if work is None:
+        work = torch.empty((2, H, W), device=Ix.device, dtype=torch.float32)
+    ta = _Taps(Avg)
+    dptr = _ptr(den) if den is not None else ctypes.c_void_p(0)
+    common = [_ptr(Ix), _ptr(Iy), _ptr(It), dptr, ctypes.c_float(alpha2), H, W, *ta.args(),
+              int(iters), _ptr(u), _ptr(v), _ptr(work[0]), _ptr(work[1])]
+    if variant is None:
+        check(lib().hmx_hs_jacobi(*common, stream_ptr()), "hs_jacobi")
+    else:
+        check(lib().hmx_hs_jacobi_cfg(*common, int(variant), int(k), stream_ptr()), "hs_jacobi")
+    return u, v
+
+
+def hs_tune(Ix, Iy, It, alpha2, Avg):
+    """Autotune (variant, k) for this image size; buffers are scratch copies."""
+    H, W = Ix.shape
+    u = torch.zeros_like(Ix)
+    v = torch.zeros_like(Ix)
+    work = torch.empty((2, H, W), device=Ix.device, dtype=torch.float32)
+    ta = _Taps(Avg)
+    vo, ko = ctypes.c_int(0), ctypes.c_int(0)
+    check(lib().hmx_hs_tune(_ptr(Ix), _ptr(Iy), _ptr(It), ctypes.c_float(alpha2), H, W,
+                            *ta.args(), _ptr(u), _ptr(v), _ptr(work[0]), _ptr(work[1]),
+                            stream_ptr(), ctypes.byref(vo), ctypes.byref(ko)), "hs_tune")
+    return vo.value, ko.value
+
+
+def hs_variants():
+    L = lib()
+    out = []
+    for i in range(L.hmx_hs_variant_count()):
+        tw, th, nt, km = (ctypes.c_int() for _ in range(4))
+        check(L.hmx_hs_variant_info(i, ctypes.byref(tw), ctypes.byref(th), ctypes.byref(nt),
+                                    ctypes.byref(km)), "variant_info")
+        out.append(dict(tile_w=tw.value, tile_h=th.value, threads=nt.value, kmax=km.value))
+    return out
+
+
+def hs_solve(I1, I2, alpha2, taps, iters, u=None, v=None, workspace=None):
+    """a9+a10: full solve from (u, v) (zeros by default); returns (u, v)."""
+    _dev(I1, "I1")
+    _dev(I2, "I2")
+    H, W = I1.shape
+    u = torch.zeros_like(I1) if u is None else _dev(u, "u")
+    v = torch.zeros_like(I1) if v is None else _dev(v, "v")
+    if workspace is None:
+        workspace = torch.empty((5, H, W), device=I1.device, dtype=torch.float32)
+    ts = [_Taps(taps[k]) for k in ("Dx", "Dy", "Dt", "Avg")]
+    args = []
+    for t in ts:
+        args += t.args()
+    check(lib().hmx_hs_solve(_ptr(I1), _ptr(I2), H, W, ctypes.c_float(alpha2), *args,
+                             int(iters), _ptr(u), _ptr(v), _ptr(workspace), stream_ptr()),
+          "hs_solve")
+    return u, v

@@ -1,0 +1,205 @@
+/*
+ * hmx.h -- C ABI of libhmx, the MI355X (gfx950) backend for Hindemith's
+ * Horn-Schunck hot path.
+ *
+ * Two groups of entry points:
+ *
+ *  1. The record API replaces the reference's Python-2 extension module
+ *     `backend` (backend.cpp) and everything under it (sm.cpp, oclgen.cpp,
+ *     fusion.cpp, lm.cpp, execute.cpp, clhelp.cpp).  It keeps the reference's
+ *     XML wire format so the Python front end stays a faithful port.
+ *  2. Device-pointer entry points: one per hot-path kernel, for torch callers
+ *     and benchmarks.  Pointers are device pointers unless stated; `stream` is
+ *     a hipStream_t (NULL = legacy default stream).  Nothing allocates device
+ *     memory behind the caller's back except where a call says so.
+ *
+ * Every function returns HMX_OK (0) or an HMX_ERR_* code; hmx_last_error()
+ * gives the message (thread-local).  Plain C types only.
+ */
+#ifndef HMX_H
+#define HMX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HMX_MAX_TAPS 32
+
+enum hmx_status {
+    HMX_OK = 0,
+    HMX_ERR_INVALID = 1,     /* bad argument, shape, dtype or program */
+    HMX_ERR_HIP = 2,         /* HIP runtime error */
+    HMX_ERR_UNSUPPORTED = 3, /* valid for the reference, not built here */
+    HMX_ERR_NOT_FOUND = 4,   /* unknown record / symbol / result */
+    HMX_ERR_STATE = 5,       /* call out of order (e.g. run before compile) */
+    HMX_ERR_TIMEOUT = 6      /* a bounded device-side wait gave up */
+};
+
+/* dtypes, as Types/primitives.cpp:4-44 names them */
+enum hmx_dtype { HMX_F32 = 0, HMX_F64 = 1, HMX_I64 = 2, HMX_I32 = 3 };
+
+/* symbol kinds (sm.h:17-263 Variable subclasses on the hot path) */
+enum hmx_kind {
+    HMX_KIND_ARRAY2D = 1,
+    HMX_KIND_ARRAY = 2,
+    HMX_KIND_SCALAR = 3,
+    HMX_KIND_STENCIL = 4,
+    HMX_KIND_SPMAT2D = 5,
+    HMX_KIND_CSRMAT = 6
+};
+
+const char *hmx_last_error(void);
+const char *hmx_version(void);
+
+/* ====================================================================== */
+/* 1. Record API                                                           */
+/* ====================================================================== */
+
+/* Select the device for records.  Replaces initbackend -> initialize_ocl
+ * (backend.cpp:669-675, clhelp.cpp:33-140); the HM_* environment mapping is
+ * done by the Python front end. */
+int hmx_init(int device);
+
+/* Parse symbol table, semantic model and constants; plan kernels; allocate
+ * device buffers.  Replaces backend.xmlcompile (backend.cpp:70-159). */
+int hmx_compile(int rec, const char *symtab_xml, const char *sm_xml,
+                const char *defines_xml);
+
+/* Point a Header symbol at caller memory for the next hmx_run.  component:
+ * 0 = data (Array/Array2D/Scalar/Stencil/SpMat2D/CSRMat), 1 = offx (Stencil,
+ * SpMat2D) or indices (CSRMat), 2 = offy or indptr.  nbytes must equal the
+ * symbol's size.  Replaces the binding loop of backend.run
+ * (backend.cpp:384-541); the caller keeps the memory alive until hmx_run
+ * returns. */
+int hmx_bind(int rec, const char *name, int component, const void *host, int64_t nbytes);
+
+/* Upload Header symbols, execute the plan, download Return values (then
+ * synchronise).  Replaces backend.run (backend.cpp:320-638). */
+int hmx_run(int rec);
+
+int hmx_result_count(int rec, int *n);
+/* shape has room for 3 entries; *name stays valid until hmx_release. */
+int hmx_result_info(int rec, int i, int *kind, int *dtype, int *ndim, int64_t *shape,
+                    const char **name);
+/* Copy return value i of the last run into host memory (createPyObject,
+ * backend.cpp:160-313, which instead wrapped backend memory). */
+int hmx_result_read(int rec, int i, void *host, int64_t nbytes);
+/* Human-readable plan: one line per launch group. */
+int hmx_plan_text(int rec, char *buf, int64_t cap);
+/* Free a record's device buffers and graphs (free_ocl, backend.cpp:646-651,
+ * was a no-op). */
+int hmx_release(int rec);
+
+/* ====================================================================== */
+/* 2. Device-pointer entry points                                          */
+/* ====================================================================== */
+
+/* a1. y = S * x with clamped (replicated) borders.  Stencil2DMVM.cpp:36-58.
+ *     x, y: H x W row-major fp32 (distinct buffers).  w/offx/offy: host
+ *     arrays of nd taps, accumulated in tap order. */
+int hmx_stencil2d(const float *x, float *y, int H, int W, int nd, const float *w,
+                  const int *offx, const int *offy, void *stream);
+
+/* a2. y = A * x, 2D DIA with the coefficient stored at the neighbour index
+ *     and out-of-range terms skipped.  SpmvDIA2D.cpp:35-55, SpMat2D.cpp:211-216.
+ *     A: nd x H x W fp32 (device); offx/offy: host int arrays of nd. */
+int hmx_dia2d_spmv(const float *A, int nd, const int *offx, const int *offy,
+                   const float *x, float *y, int H, int W, void *stream);
+
+/* a3. y = A * x, CSR with int32 indices.  SpmvCSR.cpp:32-77.
+ *     lanes_per_row: 0 = choose from nnz/M, else 1,2,4,...,64. */
+int hmx_csr_spmv(int M, int N, int64_t nnz, const int *indptr, const int *indices,
+                 const float *data, const float *x, float *y, int lanes_per_row,
+                 void *stream);
+
+/* Row-binned CSR SpMV for irregular (e.g. power-law) row lengths.  bins is a
+ * device int array of M row ids sorted into 4 length classes whose counts are
+ * given by bin_counts (host, 4 entries): rows with <=4, <=16, <=64 and >64
+ * entries.  Build it once per matrix with hmx_csr_bin_rows. */
+int hmx_csr_bin_rows(int M, const int *indptr_host, int *bins_host, int *bin_counts);
+int hmx_csr_spmv_binned(int M, const int *indptr, const int *indices, const float *data,
+                        const float *x, float *y, const int *bins, const int *bin_counts,
+                        void *stream);
+
+/* a7. out[0] = sum of x[0..n) (sum2d, Array2DSum.cpp:12-250).  Deterministic
+ *     two-pass reduction with fp64 accumulation.  partials: device scratch of
+ *     HMX_SUM_PARTIALS doubles.  out: device float. */
+#define HMX_SUM_PARTIALS 1024
+int hmx_sum2d(const float *x, int64_t n, double *partials, float *out, void *stream);
+
+/* a4-a8. Fused elementwise program over n elements (Array2DOp,
+ * Array2DScalarOp, ScalarArray2DOp, Array2DCopy, Array2DSqrt and stencil /
+ * DIA taps of Stencil2DMVM / SpmvDIA2D).  See hmx_program.h for the
+ * instruction encoding; exposed for the record runtime and tests. */
+struct hmx_instr {
+    int32_t op;   /* HMX_OP_* */
+    int32_t dst;  /* register */
+    int32_t a;    /* register or slot */
+    int32_t b;    /* register or slot */
+};
+enum hmx_op {
+    HMX_OP_LOAD = 1,    /* r[dst] = array[a][i] */
+    HMX_OP_LOADS = 2,   /* r[dst] = scalar[a] */
+    HMX_OP_STENCIL = 3, /* r[dst] = stencil[b] applied to array[a] (clamp) */
+    HMX_OP_DIA = 4,     /* r[dst] = spmat2d[b] applied to array[a] (skip) */
+    HMX_OP_ADD = 5,
+    HMX_OP_SUB = 6,
+    HMX_OP_MUL = 7,
+    HMX_OP_DIV = 8,
+    HMX_OP_SQRT = 9,    /* r[dst] = sqrt(r[a]) */
+    HMX_OP_STORE = 10,  /* out[dst][i] = r[a] */
+    HMX_OP_REDUCE = 11  /* reduction slot dst += r[a] */
+};
+
+/* a10. Canonical HS setup: Ix = Dx*I1 + Dx*I2, Iy = Dy*I1 + Dy*I2,
+ *      It = Dt*I2 - Dt*I1 (clamped stencils; one rounding per op). */
+int hmx_hs_setup(const float *I1, const float *I2, int H, int W,
+                 int ndx, const float *wx, const int *oxx, const int *oyx,
+                 int ndy, const float *wy, const int *oxy, const int *oyy,
+                 int ndt, const float *wt, const int *oxt, const int *oyt,
+                 float *Ix, float *Iy, float *It, void *stream);
+
+/* a9. `iters` fused Jacobi iterations of the canonical loop body
+ *        ubar = Avg*u; vbar = Avg*v; t = (Ix*ubar + Iy*vbar + It)/den;
+ *        u = ubar - Ix*t; v = vbar - Iy*t
+ *     with den = alpha2 + Ix*Ix + Iy*Iy recomputed in registers when den is
+ *     NULL (else read).  u, v are updated in place; work_u/work_v are H*W
+ *     device scratch planes (ping-pong).  Avg taps must lie within radius 1.
+ *     Kernel variant and iterations per launch come from the autotuner cache
+ *     (hmx_hs_tune) or a size heuristic. */
+int hmx_hs_jacobi(const float *Ix, const float *Iy, const float *It, const float *den,
+                  float alpha2, int H, int W, int nda, const float *wa, const int *oxa,
+                  const int *oya, int iters, float *u, float *v, float *work_u,
+                  float *work_v, void *stream);
+
+/* Same, with an explicit kernel variant (0..hmx_hs_variant_count()-1) and
+ * iterations per launch k (1..max for that variant). */
+int hmx_hs_jacobi_cfg(const float *Ix, const float *Iy, const float *It, const float *den,
+                      float alpha2, int H, int W, int nda, const float *wa, const int *oxa,
+                      const int *oya, int iters, float *u, float *v, float *work_u,
+                      float *work_v, int variant, int k, void *stream);
+int hmx_hs_variant_count(void);
+/* Tile width/height, threads and max k of a variant. */
+int hmx_hs_variant_info(int variant, int *tile_w, int *tile_h, int *threads, int *kmax);
+/* Time every (variant, k) candidate on the given buffers (their contents are
+ * overwritten) and cache the best for this H x W; returns the choice. */
+int hmx_hs_tune(const float *Ix, const float *Iy, const float *It, float alpha2, int H, int W,
+                int nda, const float *wa, const int *oxa, const int *oya, float *u, float *v,
+                float *work_u, float *work_v, void *stream, int *variant, int *k);
+
+/* a9+a10. Whole solve: setup, then iters Jacobi iterations from the (u, v)
+ *     passed in.  workspace: 5*H*W device floats (Ix, Iy, It, 2 ping-pong). */
+int hmx_hs_solve(const float *I1, const float *I2, int H, int W, float alpha2,
+                 int ndx, const float *wx, const int *oxx, const int *oyx,
+                 int ndy, const float *wy, const int *oxy, const int *oyy,
+                 int ndt, const float *wt, const int *oxt, const int *oyt,
+                 int nda, const float *wa, const int *oxa, const int *oya,
+                 int iters, float *u, float *v, float *workspace, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HMX_H */

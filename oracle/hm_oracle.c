@@ -1,0 +1,440 @@
+/*
+ * hm_oracle.c -- CPU restatement of Hindemith's Horn-Schunck hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/, the
+ * smoke entry point and the benchmark's cpu_baseline leg load it.  The
+ * product library (libhmx.so) never links, calls or falls back to it.
+ *
+ * Every function restates a kernel fragment of the reference (chick/hindemith)
+ * and cites the file:line it follows.  Arithmetic conventions:
+ *
+ *  - fp32 functions reproduce the generated OpenCL's rounding order: every
+ *    Python binary op of the user program is its own statement (one rounding,
+ *    no contraction across statements), while a stencil / DIA / CSR
+ *    accumulation "y += w * x" is a single OpenCL expression that the vendor
+ *    compiler may contract, restated here as fmaf(w, x, y) in tap order,
+ *    starting from y = 0.0.  Built with -ffp-contract=off so the compiler adds
+ *    no contraction of its own.
+ *  - fp64 functions run the same operation order in double; they are the
+ *    "exact" reference for endpoint-error and gradient checks.
+ *
+ * Parity pins (see tests/test_oracle_*.py): scipy.ndimage.correlate
+ * (mode='nearest'), scipy.sparse products, closed forms (constant field,
+ * linear ramp, 1x1 image), the HS fixed-point / zero / linearity identities,
+ * brute-force Python loops on tiny inputs, transpose identities and central
+ * finite differences for the gradient.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+static inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+int or_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void or_set_num_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+/* ------------------------------------------------------------------------- */
+/* a1  Stencil2DMVM: y[p] = sum_d w[d] * x[clampX(p.x+offx[d]), clampY(p.y+offy[d])]
+ *     Stencil2DMVM.cpp:36-47 (y = 0.0; for diag: clamp, y += D*x).  Row-major
+ *     H x W with element_id0 = column (Array2D.cpp:297-310).                  */
+void or_stencil2d_f32(const float *x, int H, int W, int nd, const float *w,
+                      const int *ox, const int *oy, float *y)
+{
+    #pragma omp parallel for schedule(static)
+    for (int r = 0; r < H; ++r) {
+        for (int c = 0; c < W; ++c) {
+            float acc = 0.0f;
+            for (int d = 0; d < nd; ++d) {
+                int xx = clampi(c + ox[d], 0, W - 1);
+                int yy = clampi(r + oy[d], 0, H - 1);
+                acc = fmaf(w[d], x[(int64_t)yy * W + xx], acc);
+            }
+            y[(int64_t)r * W + c] = acc;
+        }
+    }
+}
+
+void or_stencil2d_f64(const double *x, int H, int W, int nd, const double *w,
+                      const int *ox, const int *oy, double *y)
+{
+    #pragma omp parallel for schedule(static)
+    for (int r = 0; r < H; ++r) {
+        for (int c = 0; c < W; ++c) {
+            double acc = 0.0;
+            for (int d = 0; d < nd; ++d) {
+                int xx = clampi(c + ox[d], 0, W - 1);
+                int yy = clampi(r + oy[d], 0, H - 1);
+                acc += w[d] * x[(int64_t)yy * W + xx];
+            }
+            y[(int64_t)r * W + c] = acc;
+        }
+    }
+}
+
+/* Transpose of the clamped stencil, written as the literal scatter
+ * out[clamp(p + o_d)] += w_d * g[p] (serial, fp64).  Used by the gradient
+ * oracle; the GPU adjoint uses a gather form, so the two are independent.  */
+void or_stencil2d_T_f64(const double *g, int H, int W, int nd, const double *w,
+                        const int *ox, const int *oy, double *out)
+{
+    memset(out, 0, sizeof(double) * (size_t)H * W);
+    for (int r = 0; r < H; ++r)
+        for (int c = 0; c < W; ++c)
+            for (int d = 0; d < nd; ++d) {
+                int xx = clampi(c + ox[d], 0, W - 1);
+                int yy = clampi(r + oy[d], 0, H - 1);
+                out[(int64_t)yy * W + xx] += w[d] * g[(int64_t)r * W + c];
+            }
+}
+
+/* ------------------------------------------------------------------------- */
+/* a2  SpmvDIA2D: y[p] = sum_d [p+o_d in bounds] A[d][p+o_d] * x[p+o_d]
+ *     SpmvDIA2D.cpp:35-45; coefficient index SpMat2D.cpp:211-216
+ *     (element_id0 + element_id1*W + diag*W*H + offx + offy*W).             */
+void or_dia2d_f32(const float *A, int nd, const int *ox, const int *oy,
+                  const float *x, int H, int W, float *y)
+{
+    const int64_t plane = (int64_t)H * W;
+    #pragma omp parallel for schedule(static)
+    for (int r = 0; r < H; ++r) {
+        for (int c = 0; c < W; ++c) {
+            float acc = 0.0f;
+            for (int d = 0; d < nd; ++d) {
+                int xx = c + ox[d], yy = r + oy[d];
+                if (xx >= 0 && xx < W && yy >= 0 && yy < H) {
+                    int64_t q = (int64_t)yy * W + xx;
+                    acc = fmaf(A[d * plane + q], x[q], acc);
+                }
+            }
+            y[(int64_t)r * W + c] = acc;
+        }
+    }
+}
+
+void or_dia2d_f64(const double *A, int nd, const int *ox, const int *oy,
+                  const double *x, int H, int W, double *y)
+{
+    const int64_t plane = (int64_t)H * W;
+    #pragma omp parallel for schedule(static)
+    for (int r = 0; r < H; ++r) {
+        for (int c = 0; c < W; ++c) {
+            double acc = 0.0;
+            for (int d = 0; d < nd; ++d) {
+                int xx = c + ox[d], yy = r + oy[d];
+                if (xx >= 0 && xx < W && yy >= 0 && yy < H) {
+                    int64_t q = (int64_t)yy * W + xx;
+                    acc += A[d * plane + q] * x[q];
+                }
+            }
+            y[(int64_t)r * W + c] = acc;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* a3  SpmvCSR: y[i] = sum_{j in [indptr[i], indptr[i+1])} data[j]*x[indices[j]]
+ *     SpmvCSR.cpp:32-38 (one work-item per row, sequential j).  int32
+ *     indices/indptr (CSRMat.cpp:26-27).                                     */
+void or_csr_f32(int M, const int *indptr, const int *indices, const float *data,
+                const float *x, float *y)
+{
+    #pragma omp parallel for schedule(dynamic, 1024)
+    for (int i = 0; i < M; ++i) {
+        float acc = 0.0f;
+        for (int j = indptr[i]; j < indptr[i + 1]; ++j)
+            acc = fmaf(data[j], x[indices[j]], acc);
+        y[i] = acc;
+    }
+}
+
+void or_csr_f64(int M, const int *indptr, const int *indices, const double *data,
+                const double *x, double *y)
+{
+    #pragma omp parallel for schedule(dynamic, 1024)
+    for (int i = 0; i < M; ++i) {
+        double acc = 0.0;
+        for (int j = indptr[i]; j < indptr[i + 1]; ++j)
+            acc += data[j] * x[indices[j]];
+        y[i] = acc;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* a7  Array2DSum (sum2d): exact-order-free fp64 sum (Array2DSum.cpp:12-119
+ *     reduces in fp32 over a work-group tree; the oracle gives the value the
+ *     tree approximates, with |err| <= n*eps32*sum|x| as the test bound).    */
+double or_sum_f64(const float *x, int64_t n)
+{
+    double s = 0.0;
+    #pragma omp parallel for reduction(+:s) schedule(static)
+    for (int64_t i = 0; i < n; ++i) s += (double)x[i];
+    return s;
+}
+
+/* ------------------------------------------------------------------------- */
+/* a10 HS setup of the canonical program:
+ *       Ix = Dx*I1 + Dx*I2 ; Iy = Dy*I1 + Dy*I2 ; It = Dt*I2 - Dt*I1
+ *     each "S * X" a Stencil2DMVM (clamp), each +/- an Array2DOp
+ *     (Array2DOp.cpp:27-53), one rounding per op.                            */
+typedef struct {
+    int nd;
+    const float *w;
+    const int *ox;
+    const int *oy;
+} or_st32;
+
+static inline float st_apply_f32(const float *x, int H, int W, int r, int c,
+                                 int nd, const float *w, const int *ox, const int *oy)
+{
+    float acc = 0.0f;
+    for (int d = 0; d < nd; ++d) {
+        int xx = clampi(c + ox[d], 0, W - 1);
+        int yy = clampi(r + oy[d], 0, H - 1);
+        acc = fmaf(w[d], x[(int64_t)yy * W + xx], acc);
+    }
+    return acc;
+}
+
+static inline double st_apply_f64(const double *x, int H, int W, int r, int c,
+                                  int nd, const double *w, const int *ox, const int *oy)
+{
+    double acc = 0.0;
+    for (int d = 0; d < nd; ++d) {
+        int xx = clampi(c + ox[d], 0, W - 1);
+        int yy = clampi(r + oy[d], 0, H - 1);
+        acc += w[d] * x[(int64_t)yy * W + xx];
+    }
+    return acc;
+}
+
+void or_hs_setup_f32(const float *I1, const float *I2, int H, int W,
+                     int ndx, const float *wx, const int *oxx, const int *oyx,
+                     int ndy, const float *wy, const int *oxy, const int *oyy,
+                     int ndt, const float *wt, const int *oxt, const int *oyt,
+                     float *Ix, float *Iy, float *It)
+{
+    #pragma omp parallel for schedule(static)
+    for (int r = 0; r < H; ++r)
+        for (int c = 0; c < W; ++c) {
+            int64_t p = (int64_t)r * W + c;
+            float a1 = st_apply_f32(I1, H, W, r, c, ndx, wx, oxx, oyx);
+            float a2 = st_apply_f32(I2, H, W, r, c, ndx, wx, oxx, oyx);
+            Ix[p] = a1 + a2;
+            float b1 = st_apply_f32(I1, H, W, r, c, ndy, wy, oxy, oyy);
+            float b2 = st_apply_f32(I2, H, W, r, c, ndy, wy, oxy, oyy);
+            Iy[p] = b1 + b2;
+            float t2 = st_apply_f32(I2, H, W, r, c, ndt, wt, oxt, oyt);
+            float t1 = st_apply_f32(I1, H, W, r, c, ndt, wt, oxt, oyt);
+            It[p] = t2 - t1;
+        }
+}
+
+void or_hs_setup_f64(const double *I1, const double *I2, int H, int W,
+                     int ndx, const double *wx, const int *oxx, const int *oyx,
+                     int ndy, const double *wy, const int *oxy, const int *oyy,
+                     int ndt, const double *wt, const int *oxt, const int *oyt,
+                     double *Ix, double *Iy, double *It)
+{
+    #pragma omp parallel for schedule(static)
+    for (int r = 0; r < H; ++r)
+        for (int c = 0; c < W; ++c) {
+            int64_t p = (int64_t)r * W + c;
+            Ix[p] = st_apply_f64(I1, H, W, r, c, ndx, wx, oxx, oyx) +
+                    st_apply_f64(I2, H, W, r, c, ndx, wx, oxx, oyx);
+            Iy[p] = st_apply_f64(I1, H, W, r, c, ndy, wy, oxy, oyy) +
+                    st_apply_f64(I2, H, W, r, c, ndy, wy, oxy, oyy);
+            It[p] = st_apply_f64(I2, H, W, r, c, ndt, wt, oxt, oyt) -
+                    st_apply_f64(I1, H, W, r, c, ndt, wt, oxt, oyt);
+        }
+}
+
+/* a9  HS Jacobi iterations of the canonical program (Templates.py:57-71 loop,
+ *     body lowered to Stencil2DMVM + Array2DOp chain + Array2DCopy):
+ *        den  = alpha2 + Ix*Ix + Iy*Iy          (before the loop)
+ *        ubar = Avg*u ; vbar = Avg*v
+ *        t    = (Ix*ubar + Iy*vbar + It) / den
+ *        u    = ubar - Ix*t ; v = vbar - Iy*t
+ *     Jacobi semantics (all stencil reads see the previous iterate) as the
+ *     reference's kernel split enforces.  u, v are updated in place.
+ *     Python precedence: ((Ix*ubar) + (Iy*vbar)) + It and
+ *     (alpha2 + (Ix*Ix)) + (Iy*Iy).                                          */
+void or_hs_jacobi_f32(const float *Ix, const float *Iy, const float *It,
+                      float alpha2, int H, int W, int nda, const float *wa,
+                      const int *oxa, const int *oya, int iters,
+                      float *u, float *v)
+{
+    size_t n = (size_t)H * W;
+    float *un = (float *)malloc(n * sizeof(float));
+    float *vn = (float *)malloc(n * sizeof(float));
+    float *den = (float *)malloc(n * sizeof(float));
+    #pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < (int64_t)n; ++p) {
+        float a = Ix[p] * Ix[p];
+        float b = alpha2 + a;
+        float c = Iy[p] * Iy[p];
+        den[p] = b + c;
+    }
+    for (int it = 0; it < iters; ++it) {
+        #pragma omp parallel for schedule(static)
+        for (int r = 0; r < H; ++r)
+            for (int c = 0; c < W; ++c) {
+                int64_t p = (int64_t)r * W + c;
+                float ub = st_apply_f32(u, H, W, r, c, nda, wa, oxa, oya);
+                float vb = st_apply_f32(v, H, W, r, c, nda, wa, oxa, oya);
+                float m1 = Ix[p] * ub;
+                float m2 = Iy[p] * vb;
+                float s1 = m1 + m2;
+                float s2 = s1 + It[p];
+                float t = s2 / den[p];
+                float ix_t = Ix[p] * t;
+                float iy_t = Iy[p] * t;
+                un[p] = ub - ix_t;
+                vn[p] = vb - iy_t;
+            }
+        memcpy(u, un, n * sizeof(float));
+        memcpy(v, vn, n * sizeof(float));
+    }
+    free(un); free(vn); free(den);
+}
+
+void or_hs_jacobi_f64(const double *Ix, const double *Iy, const double *It,
+                      double alpha2, int H, int W, int nda, const double *wa,
+                      const int *oxa, const int *oya, int iters,
+                      double *u, double *v)
+{
+    size_t n = (size_t)H * W;
+    double *un = (double *)malloc(n * sizeof(double));
+    double *vn = (double *)malloc(n * sizeof(double));
+    for (int it = 0; it < iters; ++it) {
+        #pragma omp parallel for schedule(static)
+        for (int r = 0; r < H; ++r)
+            for (int c = 0; c < W; ++c) {
+                int64_t p = (int64_t)r * W + c;
+                double den = alpha2 + Ix[p] * Ix[p] + Iy[p] * Iy[p];
+                double ub = st_apply_f64(u, H, W, r, c, nda, wa, oxa, oya);
+                double vb = st_apply_f64(v, H, W, r, c, nda, wa, oxa, oya);
+                double t = (Ix[p] * ub + Iy[p] * vb + It[p]) / den;
+                un[p] = ub - Ix[p] * t;
+                vn[p] = vb - Iy[p] * t;
+            }
+        memcpy(u, un, n * sizeof(double));
+        memcpy(v, vn, n * sizeof(double));
+    }
+    free(un); free(vn);
+}
+
+/* ------------------------------------------------------------------------- */
+/* a13 Reverse-mode gradient of  L(u_N, v_N)  for the HS solve, in fp64.
+ *     Forward:  setup (a10), den = alpha^2 + Ix^2 + Iy^2, N Jacobi steps
+ *     from (u0, v0).  Given gu = dL/du_N, gv = dL/dv_N, returns dL/dI1,
+ *     dL/dI2, dL/dalpha (and dL/du0, dL/dv0 in gu, gv on exit).
+ *     All intermediate iterates are stored (the oracle is only run on sizes
+ *     where that fits).  Per step, with s = -(Ix*gu + Iy*gv)/den:
+ *        g_ubar = gu + Ix*s ; g_vbar = gv + Iy*s
+ *        gIx += -gu*t + s*ubar - 2*s*t*Ix ; gIy likewise ; gIt += s
+ *        galpha2 += sum(-s*t) ;  gu <- Avg^T g_ubar ; gv <- Avg^T g_vbar
+ *     then gI1 = Dx^T gIx + Dy^T gIy - Dt^T gIt, gI2 = ... + Dt^T gIt.       */
+void or_hs_grad_f64(const double *I1, const double *I2, int H, int W, double alpha,
+                    int ndx, const double *wx, const int *oxx, const int *oyx,
+                    int ndy, const double *wy, const int *oxy, const int *oyy,
+                    int ndt, const double *wt, const int *oxt, const int *oyt,
+                    int nda, const double *wa, const int *oxa, const int *oya,
+                    int iters, const double *u0, const double *v0,
+                    double *gu, double *gv,
+                    double *gI1, double *gI2, double *galpha)
+{
+    size_t n = (size_t)H * W;
+    double *Ix = (double *)malloc(n * sizeof(double));
+    double *Iy = (double *)malloc(n * sizeof(double));
+    double *It = (double *)malloc(n * sizeof(double));
+    double *den = (double *)malloc(n * sizeof(double));
+    double *us = (double *)malloc(n * sizeof(double) * (size_t)(iters + 1));
+    double *vs = (double *)malloc(n * sizeof(double) * (size_t)(iters + 1));
+    double *gIx = (double *)calloc(n, sizeof(double));
+    double *gIy = (double *)calloc(n, sizeof(double));
+    double *gIt = (double *)calloc(n, sizeof(double));
+    double *gub = (double *)malloc(n * sizeof(double));
+    double *gvb = (double *)malloc(n * sizeof(double));
+    double *tmp = (double *)malloc(n * sizeof(double));
+    double alpha2 = alpha * alpha;
+    or_hs_setup_f64(I1, I2, H, W, ndx, wx, oxx, oyx, ndy, wy, oxy, oyy, ndt, wt, oxt, oyt,
+                    Ix, Iy, It);
+    for (size_t p = 0; p < n; ++p) den[p] = alpha2 + Ix[p] * Ix[p] + Iy[p] * Iy[p];
+    memcpy(us, u0, n * sizeof(double));
+    memcpy(vs, v0, n * sizeof(double));
+    for (int k = 0; k < iters; ++k) {
+        memcpy(us + n * (k + 1), us + n * k, n * sizeof(double));
+        memcpy(vs + n * (k + 1), vs + n * k, n * sizeof(double));
+        or_hs_jacobi_f64(Ix, Iy, It, alpha2, H, W, nda, wa, oxa, oya, 1,
+                         us + n * (k + 1), vs + n * (k + 1));
+    }
+    double ga2 = 0.0;
+    for (int k = iters - 1; k >= 0; --k) {
+        const double *uk = us + n * k, *vk = vs + n * k;
+        for (int r = 0; r < H; ++r)
+            for (int c = 0; c < W; ++c) {
+                size_t p = (size_t)r * W + c;
+                double ub = st_apply_f64(uk, H, W, r, c, nda, wa, oxa, oya);
+                double vb = st_apply_f64(vk, H, W, r, c, nda, wa, oxa, oya);
+                double t = (Ix[p] * ub + Iy[p] * vb + It[p]) / den[p];
+                double s = -(Ix[p] * gu[p] + Iy[p] * gv[p]) / den[p];
+                gub[p] = gu[p] + Ix[p] * s;
+                gvb[p] = gv[p] + Iy[p] * s;
+                gIx[p] += -gu[p] * t + s * ub - 2.0 * s * t * Ix[p];
+                gIy[p] += -gv[p] * t + s * vb - 2.0 * s * t * Iy[p];
+                gIt[p] += s;
+                ga2 += -s * t;
+            }
+        or_stencil2d_T_f64(gub, H, W, nda, wa, oxa, oya, gu);
+        or_stencil2d_T_f64(gvb, H, W, nda, wa, oxa, oya, gv);
+    }
+    /* den = alpha^2 + ... : dL/dalpha = 2*alpha * dL/dalpha2 */
+    *galpha = 2.0 * alpha * ga2;
+    or_stencil2d_T_f64(gIx, H, W, ndx, wx, oxx, oyx, gI1);
+    or_stencil2d_T_f64(gIy, H, W, ndy, wy, oxy, oyy, tmp);
+    for (size_t p = 0; p < n; ++p) gI1[p] += tmp[p];
+    memcpy(gI2, gI1, n * sizeof(double));
+    or_stencil2d_T_f64(gIt, H, W, ndt, wt, oxt, oyt, tmp);
+    for (size_t p = 0; p < n; ++p) { gI1[p] -= tmp[p]; gI2[p] += tmp[p]; }
+    free(Ix); free(Iy); free(It); free(den); free(us); free(vs);
+    free(gIx); free(gIy); free(gIt); free(gub); free(gvb); free(tmp);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Full fp32 solve as one call: setup (a10) + den + N iterations (a9), u/v in
+ * place.  This is the CPU baseline the benchmark times (OpenMP over rows). */
+void or_hs_solve_f32(const float *I1, const float *I2, int H, int W, float alpha2,
+                     int ndx, const float *wx, const int *oxx, const int *oyx,
+                     int ndy, const float *wy, const int *oxy, const int *oyy,
+                     int ndt, const float *wt, const int *oxt, const int *oyt,
+                     int nda, const float *wa, const int *oxa, const int *oya,
+                     int iters, float *u, float *v)
+{
+    size_t n = (size_t)H * W;
+    float *Ix = (float *)malloc(n * sizeof(float));
+    float *Iy = (float *)malloc(n * sizeof(float));
+    float *It = (float *)malloc(n * sizeof(float));
+    or_hs_setup_f32(I1, I2, H, W, ndx, wx, oxx, oyx, ndy, wy, oxy, oyy, ndt, wt, oxt, oyt,
+                    Ix, Iy, It);
+    or_hs_jacobi_f32(Ix, Iy, It, alpha2, H, W, nda, wa, oxa, oya, iters, u, v);
+    free(Ix); free(Iy); free(It);
+}

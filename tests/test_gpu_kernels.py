@@ -1,0 +1,188 @@
+"""GPU parity of the hot-path kernels against the CPU oracle (device API).
+
+Bit-exact where the kernel performs the reference's operations in the same
+order (stencil, DIA, single-lane CSR, HS setup, every HS Jacobi variant);
+within stated bounds where the reduction order differs (multi-lane CSR, sum2d).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from hindemith_amd import synthetic as S
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+TAPS = S.hs_stencil_taps()
+WIDE = (np.array([0.5, -0.25, 0.125, 1.0, 0.0625], np.float32),
+        np.array([0, -5, 3, 1, 0], np.int32), np.array([0, 2, -4, 1, 6], np.int32))
+SHAPES = [(1, 1), (1, 37), (29, 1), (37, 41), (64, 64), (100, 333), (257, 130)]
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def K():
+    from hindemith_amd import kernels
+    return kernels
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("st", ["Avg", "Dx", "Dt", "Lap5", "Wide"])
+def test_stencil2d_bitexact(K, shape, st):
+    rng = np.random.default_rng(hash((shape, st)) % 2**32)
+    x = rng.uniform(-1, 1, shape).astype(np.float32)
+    taps = {"Lap5": S.laplacian5_taps(), "Wide": WIDE}.get(st) or TAPS[st]
+    y = host(K.stencil2d(dev(x), taps))
+    ref = oracle.stencil2d(x, taps)
+    assert np.array_equal(y, ref)
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (37, 41), (256, 256), (99, 517)])
+def test_dia2d_bitexact(K, shape):
+    rng = np.random.default_rng(1)
+    H, W = shape
+    w, ox, oy = S.laplacian5_taps()
+    A = rng.normal(size=(5, H, W)).astype(np.float32)
+    x = rng.uniform(-1, 1, shape).astype(np.float32)
+    y = host(K.dia2d_spmv(dev(A), ox, oy, dev(x)))
+    assert np.array_equal(y, oracle.dia2d(A, ox, oy, x))
+
+
+def _random_csr(M, N, rng, max_len=40, empty_frac=0.1):
+    lens = rng.integers(0, max_len + 1, M)
+    lens[rng.random(M) < empty_frac] = 0
+    indptr = np.zeros(M + 1, np.int32)
+    indptr[1:] = np.cumsum(lens)
+    nnz = int(indptr[-1])
+    indices = rng.integers(0, N, nnz).astype(np.int32)
+    data = rng.normal(size=nnz).astype(np.float32)
+    return indptr, indices, data
+
+
+@pytest.mark.parametrize("lanes", [1, 4, 16, 64, 0])
+def test_csr_spmv(K, lanes):
+    rng = np.random.default_rng(7)
+    M, N = 3001, 2500
+    indptr, indices, data = _random_csr(M, N, rng)
+    x = rng.uniform(-1, 1, N).astype(np.float32)
+    y = host(K.csr_spmv(dev(indptr), dev(indices), dev(data), dev(x), lanes_per_row=lanes))
+    if lanes == 1:  # one lane per row: same sequential fma chain as the reference kernel
+        assert np.array_equal(y, oracle.csr_spmv(indptr, indices, data, x))
+    ref = oracle.csr_spmv(indptr, indices, data, x, dtype=np.float64)
+    absdot = oracle.csr_spmv(indptr, indices, np.abs(data), np.abs(x), dtype=np.float64)
+    lens = np.diff(indptr)
+    bound = (lens + 2) * np.finfo(np.float32).eps * absdot + 1e-30
+    assert np.all(np.abs(y - ref) <= bound)
+
+
+def test_csr_binned_matches_vector(K):
+    rng = np.random.default_rng(3)
+    M, N = 5000, 5000
+    # power-law-ish lengths 1..512
+    lens = np.minimum(512, np.floor(rng.pareto(0.8, M) + 1)).astype(np.int64)
+    indptr = np.zeros(M + 1, np.int32)
+    indptr[1:] = np.cumsum(lens)
+    nnz = int(indptr[-1])
+    indices = rng.integers(0, N, nnz).astype(np.int32)
+    data = rng.normal(size=nnz).astype(np.float32)
+    x = rng.uniform(-1, 1, N).astype(np.float32)
+    bins = K.CSRBins(indptr, "cuda")
+    y = host(K.csr_spmv_binned(dev(indptr), dev(indices), dev(data), dev(x), bins))
+    ref = oracle.csr_spmv(indptr, indices, data, x, dtype=np.float64)
+    absdot = oracle.csr_spmv(indptr, indices, np.abs(data), np.abs(x), dtype=np.float64)
+    assert np.all(np.abs(y - ref) <= (lens + 2) * np.finfo(np.float32).eps * absdot + 1e-30)
+
+
+@pytest.mark.parametrize("n", [0, 1, 1000, 2048 * 2048 + 17])
+def test_sum2d(K, n):
+    rng = np.random.default_rng(n)
+    x = rng.uniform(-1, 1, n).astype(np.float32)
+    s = host(K.sum2d(dev(x)))[0]
+    ref = float(np.sum(x.astype(np.float64)))
+    # fp64 accumulation: only the final rounding to fp32 remains
+    assert abs(s - ref) <= 1e-6 * max(1.0, np.sum(np.abs(x.astype(np.float64))) * 1e-3)
+    assert s == np.float32(ref) or abs(s - ref) <= abs(ref) * 2 ** -23 + 1e-30
+
+
+def test_sum2d_deterministic(K):
+    x = dev(np.random.default_rng(5).normal(size=3_000_001).astype(np.float32))
+    a = host(K.sum2d(x))
+    b = host(K.sum2d(x))
+    assert a.tobytes() == b.tobytes()
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (2, 3), (37, 41), (256, 256), (200, 333)])
+def test_hs_setup_bitexact(K, shape):
+    I1, I2, _, _ = S.hs_pair(*shape, seed=11)
+    Ix, Iy, It = (host(t) for t in K.hs_setup(dev(I1), dev(I2), TAPS["Dx"], TAPS["Dy"],
+                                              TAPS["Dt"]))
+    rx, ry, rt = oracle.hs_setup(I1, I2, TAPS["Dx"], TAPS["Dy"], TAPS["Dt"])
+    assert np.array_equal(Ix, rx) and np.array_equal(Iy, ry) and np.array_equal(It, rt)
+
+
+def _variants(K):
+    out = []
+    for vi, v in enumerate(K.hs_variants()):
+        ks = sorted({1, v["kmax"], max(1, v["kmax"] - 1)})
+        out += [(vi, k) for k in ks]
+    return out
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (3, 500), (37, 41), (130, 257), (256, 256)])
+def test_hs_jacobi_all_variants_bitexact(K, shape):
+    I1, I2, _, _ = S.hs_pair(*shape, seed=3)
+    Ix, Iy, It = oracle.hs_setup(I1, I2, TAPS["Dx"], TAPS["Dy"], TAPS["Dt"])
+    iters = 13
+    ref_u, ref_v = oracle.hs_jacobi(Ix, Iy, It, 1.0, TAPS["Avg"], iters)
+    dIx, dIy, dIt = dev(Ix), dev(Iy), dev(It)
+    for vi, k in _variants(K):
+        u = torch.zeros(shape, device="cuda")
+        v = torch.zeros(shape, device="cuda")
+        K.hs_jacobi(dIx, dIy, dIt, 1.0, TAPS["Avg"], iters, u, v, variant=vi, k=k)
+        assert np.array_equal(host(u), ref_u), (vi, k)
+        assert np.array_equal(host(v), ref_v), (vi, k)
+
+
+def test_hs_jacobi_stored_den_and_nonzero_start(K):
+    I1, I2, _, _ = S.hs_pair(96, 160, seed=4)
+    Ix, Iy, It = oracle.hs_setup(I1, I2, TAPS["Dx"], TAPS["Dy"], TAPS["Dt"])
+    a2 = np.float32(0.37)
+    den = ((a2 + Ix * Ix) + Iy * Iy).astype(np.float32)
+    rng = np.random.default_rng(0)
+    u0 = rng.normal(size=Ix.shape).astype(np.float32)
+    v0 = rng.normal(size=Ix.shape).astype(np.float32)
+    ref_u, ref_v = oracle.hs_jacobi(Ix, Iy, It, a2, TAPS["Avg"], 9, u0, v0)
+    u, v = dev(u0), dev(v0)
+    K.hs_jacobi(dev(Ix), dev(Iy), dev(It), float(a2), TAPS["Avg"], 9, u, v, den=dev(den))
+    assert np.array_equal(host(u), ref_u) and np.array_equal(host(v), ref_v)
+
+
+def test_hs_solve_config1_parity(K):
+    """256x256 pair, constant flow (1.5,-0.75), alpha=1, 100 iterations."""
+    I1, I2, fx, fy = S.hs_pair(256, 256, seed=0)
+    u, v = K.hs_solve(dev(I1), dev(I2), 1.0, TAPS, 100)
+    u, v = host(u), host(v)
+    ru, rv = oracle.hs_solve(I1, I2, 1.0, TAPS, 100)
+    assert np.array_equal(u, ru) and np.array_equal(v, rv)
+    u64, v64 = oracle.np_hs_solve_f64(I1, I2, 1.0, TAPS, 100)
+    assert np.max(np.abs(u - u64)) <= 1e-4 and np.max(np.abs(v - v64)) <= 1e-4
+    epe = oracle.endpoint_error(u, v, fx, fy)
+    print(f"config1 EPE after 100 iterations: {epe:.4f} px")
+
+
+def test_hs_tune_then_solve(K):
+    I1, I2, _, _ = S.hs_pair(300, 500, seed=2)
+    Ix, Iy, It = K.hs_setup(dev(I1), dev(I2), TAPS["Dx"], TAPS["Dy"], TAPS["Dt"])
+    vi, k = K.hs_tune(Ix, Iy, It, 1.0, TAPS["Avg"])
+    assert 0 <= vi < len(K.hs_variants()) and k >= 1
+    u, v = K.hs_solve(dev(I1), dev(I2), 1.0, TAPS, 37)
+    ru, rv = oracle.hs_solve(I1, I2, 1.0, TAPS, 37)
+    assert np.array_equal(host(u), ru) and np.array_equal(host(v), rv)

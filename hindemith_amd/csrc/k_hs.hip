@@ -350,8 +350,8 @@ extern "C" int hmx_hs_tune(const float *Ix, const float *Iy, const float *It, fl
         }
         if (rc) break;
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     if (rc) return rc;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);

@@ -23,7 +23,9 @@
 //     by the fused, temporally blocked HS kernel (k_hs.hip).
 #include "program.h"
 
+#include <algorithm>
 #include <cctype>
+#include <functional>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -614,6 +616,7 @@ class Planner {
             g.out_syms.push_back(s);
             g.code.push_back(in);
         }
+        ssa_[(int)R_.groups.size()] = g.code;  // kept for loop matching
         int rc = regalloc(g);
         if (rc) return rc;
         Node n;
@@ -638,6 +641,12 @@ class Planner {
         for (int r : it->second)
             if (r <= fw->second || r > g.s1) return true;
         return false;
+    }
+
+    size_t count_live(const Group &g) {
+        size_t n = 0;
+        for (int s : g.written) n += live_out(s, g);
+        return n;
     }
 
     int new_reg(Group &g) { return g.nreg++; }
@@ -717,14 +726,15 @@ class Planner {
             if (nb && g.written.count(st.args[2])) split = true;
             for (int a : rd)
                 if (R_.syms[a].kind == HMX_KIND_SCALAR && g.red_out.count(a)) split = true;
-            if (g.code.size() + g.written.size() + 8 > (size_t)PROG_MAX_INSTR) split = true;
+            const size_t nlive = count_live(g);
+            if (g.code.size() + nlive + 8 > (size_t)PROG_MAX_INSTR) split = true;
             if (g.in_syms.size() + 3 > (size_t)PROG_MAX_IN) split = true;
             if (g.scal_syms.size() + 2 > (size_t)PROG_MAX_SCAL) split = true;
             if (nb && g.opnd_syms.size() + 1 > (size_t)PROG_MAX_OPND && !g.opnd_slot.count(st.args[1]))
                 split = true;
             if ((f == "Array2DSum" || f == "ArraySum") && g.red_syms.size() >= (size_t)PROG_MAX_RED)
                 split = true;
-            if (g.written.size() + 1 > (size_t)PROG_MAX_OUT) split = true;
+            if (nlive + 1 > (size_t)PROG_MAX_OUT) split = true;
         }
         if (split) {
             int rc = flush(out);
@@ -840,7 +850,6 @@ class Planner {
             auto it = def.find(r);
             return it == def.end() ? nullptr : it->second;
         };
-        auto is = [&](int r, int op) { const hmx_instr *d = D(r); return d && d->op == op; };
         auto loaded = [&](int r) -> int {  // sym loaded into r, or -1
             const hmx_instr *d = D(r);
             return d && d->op == HMX_OP_LOAD ? g.in_syms[d->a] : -1;
@@ -852,37 +861,46 @@ class Planner {
                 else { sv = g.out_syms[in.dst]; rv = in.a; }
             }
         if (su < 0 || sv < 0 || su == sv) return loop;
+        // MUL(LOAD X, r) in either operand order (fp32 multiply commutes)
+        auto mul_load = [&](int r, int &X, int &other) -> bool {
+            const hmx_instr *m = D(r);
+            if (!m || m->op != HMX_OP_MUL) return false;
+            if (loaded(m->a) >= 0 && loaded(m->b) < 0) { X = loaded(m->a); other = m->b; return true; }
+            if (loaded(m->b) >= 0 && loaded(m->a) < 0) { X = loaded(m->b); other = m->a; return true; }
+            return false;
+        };
         // U <- SUB(S(U), MUL(X, T))
         auto match_upd = [&](int r, int self, int &stencil, int &X, int &T, int &Sreg) -> bool {
             const hmx_instr *sub = D(r);
             if (!sub || sub->op != HMX_OP_SUB) return false;
             const hmx_instr *st = D(sub->a);
             if (!st || st->op != HMX_OP_STENCIL || g.in_syms[st->a] != self) return false;
-            const hmx_instr *m = D(sub->b);
-            if (!m || m->op != HMX_OP_MUL) return false;
-            X = loaded(m->a);
-            T = m->b;
+            if (!mul_load(sub->b, X, T)) return false;
             stencil = g.opnd_syms[st->b];
             Sreg = sub->a;
-            return X >= 0;
+            return true;
         };
         int stu, stv, Xu, Xv, Tu, Tv, Su, Sv;
         if (!match_upd(ru, su, stu, Xu, Tu, Su) || !match_upd(rv, sv, stv, Xv, Tv, Sv))
             return loop;
         if (stu != stv || Tu != Tv) return loop;
-        // T = DIV(ADD(ADD(MUL(Xu, Su), MUL(Xv, Sv)), It), Den)
+        // T = DIV(ADD(ADD(MUL(Xu, Su), MUL(Xv, Sv)), It), Den), adds in either order
         const hmx_instr *dv = D(Tu);
         if (!dv || dv->op != HMX_OP_DIV) return loop;
         int den = loaded(dv->b);
         const hmx_instr *a2 = D(dv->a);
         if (den < 0 || !a2 || a2->op != HMX_OP_ADD) return loop;
-        int it = loaded(a2->b);
-        const hmx_instr *a1 = D(a2->a);
+        int it = loaded(a2->b), inner = a2->a;
+        if (it < 0) { it = loaded(a2->a); inner = a2->b; }
+        const hmx_instr *a1 = D(inner);
         if (it < 0 || !a1 || a1->op != HMX_OP_ADD) return loop;
-        const hmx_instr *m1 = D(a1->a), *m2 = D(a1->b);
-        if (!m1 || !m2 || m1->op != HMX_OP_MUL || m2->op != HMX_OP_MUL) return loop;
-        if (loaded(m1->a) != Xu || m1->b != Su || loaded(m2->a) != Xv || m2->b != Sv) return loop;
-        (void)is;
+        auto term = [&](int r, int X, int S) {
+            int x, o;
+            return mul_load(r, x, o) && x == X && o == S;
+        };
+        const bool sums = (term(a1->a, Xu, Su) && term(a1->b, Xv, Sv)) ||
+                          (term(a1->b, Xu, Su) && term(a1->a, Xv, Sv));
+        if (!sums) return loop;
         std::set<int> ro = {Xu, Xv, it, den};
         if (ro.count(su) || ro.count(sv) || g.written.count(Xu) || g.written.count(Xv) ||
             g.written.count(it) || g.written.count(den))
@@ -897,16 +915,7 @@ class Planner {
         return hs;
     }
 
-  public:
-    std::map<int, std::vector<hmx_instr>> ssa_;  // group index -> SSA code (for matching)
-    int flush_keep_ssa(std::vector<Node> &out) { return flush(out); }
-};
-
-// The SSA copy is taken in flush() before register allocation.  To keep the
-// Planner simple the copy is made by a subclass hook here.
-class RecordingPlanner : public Planner {
-  public:
-    using Planner::Planner;
+    std::map<int, std::vector<hmx_instr>> ssa_;  // group index -> SSA code
 };
 
 // ---------------------------------------------------------------------------
@@ -939,34 +948,37 @@ static void mark_buffers(Record &R, const std::vector<Node> &nodes) {
     }
 }
 
-static int allocate(Record &R) {
+// Byte size of each component (data, offx/indices, offy/indptr).
+static void size_symbols(Record &R) {
     for (Sym &s : R.syms) {
-        int rc = HMX_OK;
         switch (s.kind) {
-            case HMX_KIND_ARRAY2D: case HMX_KIND_ARRAY: {
-                s.bytes[0] = (int64_t)s.H * s.W * 4;
-                if (s.header || s.needs_buffer) rc = dalloc(R, &s.dev[0], s.bytes[0]);
-                if (!rc && s.renamed) rc = dalloc(R, &s.alt, s.bytes[0]);
-                break;
-            }
-            case HMX_KIND_SCALAR:
-                s.bytes[0] = dtype_size(s.dtype);
-                rc = dalloc(R, &s.dev[0], 8);
-                break;
+            case HMX_KIND_ARRAY2D: case HMX_KIND_ARRAY: s.bytes[0] = (int64_t)s.H * s.W * 4; break;
+            case HMX_KIND_SCALAR: s.bytes[0] = dtype_size(s.dtype); break;
             case HMX_KIND_STENCIL:
-                s.bytes[0] = s.ndiags * 4;
-                s.bytes[1] = s.bytes[2] = s.ndiags * 4;
-                for (int k = 0; k < 3 && !rc; ++k) rc = dalloc(R, &s.dev[k], s.bytes[k]);
+                s.bytes[0] = s.bytes[1] = s.bytes[2] = s.ndiags * 4;
                 break;
             case HMX_KIND_SPMAT2D:
                 s.bytes[0] = s.ndiags * s.dim0 * s.dim1 * 4;
                 s.bytes[1] = s.bytes[2] = s.ndiags * 4;
-                for (int k = 0; k < 3 && !rc; ++k) rc = dalloc(R, &s.dev[k], s.bytes[k]);
                 break;
             case HMX_KIND_CSRMAT:
-                s.bytes[0] = s.nnz * 4;
-                s.bytes[1] = s.nnz * 4;
+                s.bytes[0] = s.bytes[1] = s.nnz * 4;
                 s.bytes[2] = (s.dim0 + 1) * 4;
+                break;
+        }
+    }
+}
+
+static int allocate(Record &R) {
+    for (Sym &s : R.syms) {
+        int rc = HMX_OK;
+        switch (s.kind) {
+            case HMX_KIND_ARRAY2D: case HMX_KIND_ARRAY:
+                if (s.header || s.needs_buffer) rc = dalloc(R, &s.dev[0], s.bytes[0]);
+                if (!rc && s.renamed) rc = dalloc(R, &s.alt, s.bytes[0]);
+                break;
+            case HMX_KIND_SCALAR: rc = dalloc(R, &s.dev[0], 8); break;
+            default:
                 for (int k = 0; k < 3 && !rc; ++k) rc = dalloc(R, &s.dev[k], s.bytes[k]);
                 break;
         }
@@ -1174,9 +1186,9 @@ static int check_defines(Record &R, const std::vector<Stmt> &b, bool &need2d, bo
             if (rc) return rc;
             continue;
         }
-        if (st.filter == "Return" || st.filter.rfind("Scalar", 0) == 0 &&
-                                         st.filter.find("Array") == std::string::npos)
-            continue;
+        const bool scalar_only = st.filter.rfind("Scalar", 0) == 0 &&
+                                 st.filter.find("Array") == std::string::npos;
+        if (st.filter == "Return" || scalar_only) continue;
         if (st.filter.find("2D") != std::string::npos) need2d = true;
         else need1d = true;
     }
@@ -1189,16 +1201,12 @@ static int check_defines(Record &R, const std::vector<Stmt> &b, bool &need2d, bo
 using namespace hmx;
 using namespace hmx::rt;
 
-// The planner keeps an SSA copy of every program for loop matching.
-namespace hmx {
-namespace rt {
-struct PlannerWithSSA : Planner {
-    using Planner::Planner;
-};
-}  // namespace rt
-}  // namespace hmx
-
 extern "C" int hmx_init(int device) {
+    if (device == -1) {  // plan-only mode: compile and describe, never run
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_device = -1;
+        return HMX_OK;
+    }
     int n = 0;
     HMX_HIP_TRY(hipGetDeviceCount(&n));
     if (device < 0 || device >= n)
@@ -1248,15 +1256,18 @@ extern "C" int hmx_compile(int rec, const char *symtab_xml, const char *sm_xml,
     if (need1d && std::atoi(R->defines["nd1"].c_str()) != 1)
         return fail(HMX_ERR_UNSUPPORTED, "nd1 must be 1: this backend runs on one GPU");
 
-    HMX_HIP_TRY(hipSetDevice(R->device));
     collect_reads(*R, R->prog);
-    // Plan (SSA copies are recorded by a two-phase flush: the planner's
-    // flush() allocates registers, so capture SSA before by planning twice is
-    // avoided -- the planner stores SSA itself, see plan_with_ssa below).
+    size_symbols(*R);
     Planner P(*R);
     rc = P.plan_block(R->prog, R->plan, true);
     if (!rc) rc = P.flush(R->plan);
     if (rc) return rc;
+    if (R->device < 0) {  // plan-only
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_records[rec] = std::move(R);
+        return HMX_OK;
+    }
+    HMX_HIP_TRY(hipSetDevice(R->device));
     HMX_HIP_TRY(hipStreamCreateWithFlags(&R->stream, hipStreamNonBlocking));
     mark_buffers(*R, R->plan);
     rc = allocate(*R);
@@ -1291,13 +1302,17 @@ extern "C" int hmx_compile(int rec, const char *symtab_xml, const char *sm_xml,
         release_record(*R);
         return rc;
     }
-    std::lock_guard<std::mutex> lk(g_mu);
-    auto it = g_records.find(rec);
-    if (it != g_records.end()) {
-        std::lock_guard<std::mutex> lk2(it->second->mu);
-        release_record(*it->second);
+    std::unique_ptr<Record> old;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto it = g_records.find(rec);
+        if (it != g_records.end()) old = std::move(it->second);
+        g_records[rec] = std::move(R);
     }
-    g_records[rec] = std::move(R);
+    if (old) {
+        std::lock_guard<std::mutex> lk2(old->mu);
+        release_record(*old);
+    }
     return HMX_OK;
 }
 
@@ -1329,6 +1344,7 @@ extern "C" int hmx_run(int rec) {
     Record *R = find_record(rec);
     if (!R) return fail(HMX_ERR_NOT_FOUND, "no record " + std::to_string(rec));
     std::lock_guard<std::mutex> lk(R->mu);
+    if (R->device < 0) return fail(HMX_ERR_STATE, "record was compiled in plan-only mode");
     HMX_HIP_TRY(hipSetDevice(R->device));
     // upload every Header symbol (the reference re-uploads on each call,
     // Array2D.cpp:149-200)

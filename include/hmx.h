@@ -59,7 +59,9 @@ const char *hmx_version(void);
 
 /* Select the device for records.  Replaces initbackend -> initialize_ocl
  * (backend.cpp:669-675, clhelp.cpp:33-140); the HM_* environment mapping is
- * done by the Python front end. */
+ * done by the Python front end.  device = -1 selects plan-only mode: records
+ * compile and report their plan (hmx_plan_text) but cannot run; no HIP call
+ * is made, so it works on hosts without a GPU. */
 int hmx_init(int device);
 
 /* Parse symbol table, semantic model and constants; plan kernels; allocate
@@ -131,8 +133,8 @@ int hmx_sum2d(const float *x, int64_t n, double *partials, float *out, void *str
 
 /* a4-a8. Fused elementwise program over n elements (Array2DOp,
  * Array2DScalarOp, ScalarArray2DOp, Array2DCopy, Array2DSqrt and stencil /
- * DIA taps of Stencil2DMVM / SpmvDIA2D).  See hmx_program.h for the
- * instruction encoding; exposed for the record runtime and tests. */
+ * DIA taps of Stencil2DMVM / SpmvDIA2D).  The record runtime encodes its
+ * fused programs with these instructions (registers live in LDS). */
 struct hmx_instr {
     int32_t op;   /* HMX_OP_* */
     int32_t dst;  /* register */

@@ -1,0 +1,253 @@
+"""Front end (Templates/PyAST port) and planner, on the CPU (plan-only mode)."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle
+from hindemith_amd import frontend as fe
+from hindemith_amd import synthetic as S
+from hindemith_amd._lib import HindemithError
+from hindemith_amd.hindemith import SpMat2D, Stencil, defines, hm, sum2d
+
+TAPS = S.hs_stencil_taps()
+DEF2D = dict(nd20=1, nd21=1, nt20=8, nt21=8, ne20=2, ne21=2)
+DEF1D = dict(nd1=1, nt1=128, ne1=2)
+
+
+@pytest.fixture(autouse=True)
+def plan_only():
+    fe.set_plan_only(True)
+    saved = dict(defines)
+    defines.clear()
+    defines.update(DEF2D)
+    defines.update(DEF1D)
+    yield
+    defines.clear()
+    defines.update(saved)
+    fe.set_plan_only(False)
+
+
+def hs_kwargs(H=32, W=48):
+    z = np.zeros((H, W), np.float32)
+    return dict(I1=z, I2=z.copy(), Dx=Stencil(*TAPS["Dx"]), Dy=Stencil(*TAPS["Dy"]),
+                Dt=Stencil(*TAPS["Dt"]), Avg=Stencil(*TAPS["Avg"]), alpha2=np.float32(1.0),
+                u=z.copy(), v=z.copy())
+
+
+@hm
+def hs_canonical(I1, I2, Dx, Dy, Dt, Avg, alpha2, u, v):
+    Ix = Dx * I1 + Dx * I2
+    Iy = Dy * I1 + Dy * I2
+    It = Dt * I2 - Dt * I1
+    den = alpha2 + Ix * Ix + Iy * Iy
+    for i in range(100):
+        ubar = Avg * u
+        vbar = Avg * v
+        t = (Ix * ubar + Iy * vbar + It) / den
+        u = ubar - Ix * t
+        v = vbar - Iy * t
+    return u, v
+
+
+@hm
+def hs_commuted(I1, I2, Dx, Dy, Dt, Avg, alpha2, u, v):
+    Ix = Dx * I1 + Dx * I2
+    Iy = Dy * I1 + Dy * I2
+    It = Dt * I2 - Dt * I1
+    den = alpha2 + Ix * Ix + Iy * Iy
+    for i in range(20):
+        ubar = Avg * u
+        vbar = Avg * v
+        t = (It + (vbar * Iy + ubar * Ix)) / den
+        u = ubar - t * Ix
+        v = vbar - t * Iy
+    return u, v
+
+
+@hm
+def hs_with_residual(I1, I2, Dx, Dy, Dt, Avg, alpha2, u, v):
+    Ix = Dx * I1 + Dx * I2
+    Iy = Dy * I1 + Dy * I2
+    It = Dt * I2 - Dt * I1
+    den = alpha2 + Ix * Ix + Iy * Iy
+    for i in range(10):
+        ubar = Avg * u
+        vbar = Avg * v
+        t = (Ix * ubar + Iy * vbar + It) / den
+        u = ubar - Ix * t
+        v = vbar - Iy * t
+    r = Ix * u + Iy * v + It
+    res = sqrt(sum2d(r * r))
+    return u, v, res
+
+
+def test_canonical_program_plans_to_fused_kernels():
+    plan = hs_canonical.compile(**hs_kwargs())
+    lines = plan.strip().splitlines()
+    assert lines[0].startswith("program 32x48: ") and "stores {Ix,Iy,It,den}" in lines[0]
+    assert lines[1].startswith("hs_jacobi_fused x100")
+    assert lines[2] == "return u v"
+
+
+def test_commuted_operands_still_fused():
+    plan = hs_commuted.compile(**hs_kwargs())
+    assert "hs_jacobi_fused x20" in plan
+
+
+def test_residual_program_plan():
+    plan = hs_with_residual.compile(**hs_kwargs())
+    assert "hs_jacobi_fused x10" in plan
+    assert "reduce 1" in plan and "scalar program" in plan
+
+
+def test_xml_wire_format():
+    kw = hs_kwargs(8, 8)
+    rec = fe.mycompile(hs_canonical.source, kw, defines)
+    assert rec.symtab_xml.startswith("<SymbolTable><Array2D name=\"I1\" source=\"Header\"")
+    assert '<Stencil name="Avg" ndiags="8"' in rec.symtab_xml
+    assert '<Iterator ub="100"><PipeAndFilter><Filter name="Stencil2DMVM">' in rec.model_xml
+    assert '<Filter name="Return"><Name name="u"/><Name name="v"/></Filter>' in rec.model_xml
+    assert rec.consts_xml.startswith("<ConstantTable><ScalarConstant name=")
+
+
+def test_record_cache_by_signature():
+    a = fe.mycompile(hs_canonical.source, hs_kwargs(8, 8), defines, fn_key="k")
+    b = fe.mycompile(hs_canonical.source, hs_kwargs(8, 8), defines, fn_key="k")
+    c = fe.mycompile(hs_canonical.source, hs_kwargs(8, 16), defines, fn_key="k")
+    assert a is b and a is not c
+
+
+def test_write_after_stencil_read_uses_ping_pong():
+    @hm
+    def smooth(Avg, x):
+        for i in range(3):
+            x = Avg * x
+        return x
+    plan = smooth.compile(Avg=Stencil(*TAPS["Avg"]), x=np.zeros((16, 16), np.float32))
+    assert "stores {x*}" in plan  # renamed: written to the second buffer
+
+
+def test_read_after_write_through_stencil_splits():
+    @hm
+    def two(Avg, x, y):
+        z = x + y
+        w = Avg * z
+        return w
+    plan = two.compile(Avg=Stencil(*TAPS["Avg"]), x=np.zeros((16, 16), np.float32),
+                       y=np.zeros((16, 16), np.float32))
+    progs = [l for l in plan.splitlines() if l.startswith("program")]
+    assert len(progs) == 2
+
+
+def test_csr_and_dia_programs():
+    @hm
+    def spmv(A, x):
+        y = A * x
+        return y
+    A = sp.random(30, 20, density=0.2, random_state=1, format="csr", dtype=np.float32)
+    plan = spmv.compile(A=A, x=np.zeros(20, np.float32))
+    assert "csr spmv" in plan
+
+    @hm
+    def dia(M, x):
+        y = M * x
+        return y
+    w, ox, oy = S.laplacian5_taps()
+    plan = dia.compile(M=SpMat2D(np.ones((5, 8, 9), np.float32), ox, oy),
+                       x=np.zeros((8, 9), np.float32))
+    assert "1 stencil/dia taps" in plan
+
+
+def test_missing_defines_is_an_error():
+    defines.clear()
+    defines.update(DEF1D)
+    with pytest.raises(HindemithError, match="nd20"):
+        hs_canonical.compile(**hs_kwargs())
+
+
+def test_multi_device_defines_rejected():
+    defines["nd20"] = 2
+    with pytest.raises(HindemithError, match="one GPU"):
+        hs_canonical.compile(**hs_kwargs())
+
+
+def test_numeric_literal_is_a_clear_error():
+    @hm
+    def lit(x):
+        y = x * 2.0
+        return y
+    with pytest.raises(HindemithError, match="numeric literals"):
+        lit.compile(x=np.zeros((4, 4), np.float32))
+
+
+def test_loop_bound_must_be_literal():
+    @hm
+    def loop(x, n):
+        for i in range(n):
+            x = x + x
+        return x
+    with pytest.raises(HindemithError, match="integer literal"):
+        loop.compile(x=np.zeros((4, 4), np.float32), n=np.int64(3))
+
+
+def test_out_of_scope_operation():
+    @hm
+    def warp(x, u, v):
+        y = warp_img2d(x, u, v)  # noqa: F821
+        return y
+    z = np.zeros((4, 4), np.float32)
+    with pytest.raises(HindemithError, match="WarpImg2D"):
+        warp.compile(x=z, u=z, v=z)
+
+
+def test_unknown_symbol():
+    @hm
+    def f(x):
+        y = x + q  # noqa: F821
+        return y
+    with pytest.raises(HindemithError, match="unknown symbol 'q'"):
+        f.compile(x=np.zeros((4, 4), np.float32))
+
+
+def test_shape_mismatch():
+    @hm
+    def f(x, y):
+        z = x + y
+        return z
+    with pytest.raises(HindemithError, match="shape"):
+        f.compile(x=np.zeros((4, 4), np.float32), y=np.zeros((4, 5), np.float32))
+
+
+def test_device_selection_from_env():
+    assert fe.device_from_env({"HM_GPU0": "1"}) == 0
+    assert fe.device_from_env({"HM_GPU1": "1"}) == 1
+    for env in ({}, {"HM_CPU0": "1"}, {"HM_GPU01": "1"}, {"HM_CPU0_SUB1": "1"}):
+        with pytest.raises(HindemithError):
+            fe.device_from_env(env)
+
+
+def test_plan_only_record_cannot_run():
+    with pytest.raises(HindemithError, match="plan-only"):
+        hs_canonical(**hs_kwargs(8, 8))
+
+
+def test_undecorated_program_runs_on_numpy_twin():
+    """Without @hm the same body is NumPy (hlib twin) and agrees with the oracle."""
+    def hs(I1, I2, Dx, Dy, Dt, Avg, alpha2, u, v):
+        Ix = Dx * I1 + Dx * I2
+        Iy = Dy * I1 + Dy * I2
+        It = Dt * I2 - Dt * I1
+        den = alpha2 + Ix * Ix + Iy * Iy
+        for i in range(30):
+            ubar = Avg * u
+            vbar = Avg * v
+            t = (Ix * ubar + Iy * vbar + It) / den
+            u = ubar - Ix * t
+            v = vbar - Iy * t
+        return u, v
+    I1, I2, _, _ = S.hs_pair(40, 56, seed=5)
+    kw = hs_kwargs(40, 56)
+    kw.update(I1=I1, I2=I2)
+    u, v = hs(**kw)
+    ru, rv = oracle.hs_solve(I1, I2, 1.0, TAPS, 30)
+    assert np.max(np.abs(u - ru)) < 1e-5 and np.max(np.abs(v - rv)) < 1e-5

@@ -59,8 +59,17 @@ _SIGS = {
                     vp, pint, pint],
     "hmx_hs_solve": [vp, vp, c_int, c_int, c_float] + [c_int, vp, vp, vp] * 4 +
                     [c_int, vp, vp, vp, vp],
+    "hmx_hs_ckpt_count": [c_int, c_int],
+    "hmx_hs_forward_ckpt": [vp, vp, c_int, c_int, c_int, vp] + [c_int, vp, vp, vp] * 4 +
+                           [c_int, c_int, vp, vp, vp, vp, vp],
+    "hmx_hs_adjoint": [vp, vp, c_int, c_int, c_int, vp] + [c_int, vp, vp, vp] * 4 +
+                      [c_int, c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp],
+    "hmx_hs_solve_f64": [vp, vp, c_int, c_int, c_int, vp] + [c_int, vp, vp, vp] * 4 +
+                        [c_int, vp, vp, vp, vp],
 }
 _RESTYPE = {"hmx_last_error": c_char_p, "hmx_version": c_char_p}
+_RESTYPE_ARGS = {"hmx_hs_forward_workspace": ([c_int, c_int, c_int], c_int64),
+                 "hmx_hs_adjoint_workspace": ([c_int, c_int, c_int, c_int], c_int64)}
 
 _lib = None
 _load_error = None
@@ -92,6 +101,11 @@ def lib():
         fn = getattr(L, name)
         fn.argtypes = []
         fn.restype = rt
+    for name, (args, rt) in _RESTYPE_ARGS.items():
+        fn = getattr(L, name, None)
+        if fn is not None:
+            fn.argtypes = args
+            fn.restype = rt
     _lib = L
     return L
 

@@ -1,0 +1,167 @@
+// Tile body of the fused, temporally blocked Horn-Schunck Jacobi sweep,
+// shared by the single-image launch (k_hs.hip) and the persistent batched
+// launch (k_batched.hip) so both run the same per-pixel code.
+#pragma once
+
+#include "hmx_common.h"
+
+#pragma clang fp contract(off)
+
+namespace hmx {
+
+// One image of the sweep: read (u_in, v_in), write (u_out, v_out).
+struct HsImage {
+    const float *Ix, *Iy, *It, *den;  // den null: recompute from alpha2
+    const float *u_in, *v_in;
+    float *u_out, *v_out;
+    float alpha2;
+    int H, W, tiles_x;
+};
+
+// Largest averaging stencil the fused kernel takes: any radius-1 stencil with
+// distinct offsets has at most 9 taps.
+constexpr int HS_MAX_TAPS = 9;
+
+// Tap table of a launch, hoisted into registers (uniform -> SGPRs).
+template <int PITCH>
+struct HsTapTable {
+    float w[HS_MAX_TAPS];
+    int dl[HS_MAX_TAPS];  // LDS delta oy*PITCH + ox
+    int ox[HS_MAX_TAPS], oy[HS_MAX_TAPS];
+    int nd;
+    __device__ __forceinline__ explicit HsTapTable(const Taps &t) {
+        nd = t.nd;
+#pragma unroll
+        for (int d = 0; d < HS_MAX_TAPS; ++d) {
+            const bool on = d < t.nd;
+            w[d] = on ? t.w[d] : 0.0f;
+            ox[d] = on ? t.ox[d] : 0;
+            oy[d] = on ? t.oy[d] : 0;
+            dl[d] = oy[d] * PITCH + ox[d];
+        }
+    }
+};
+
+// Advance output tile t of image a by k (<= KMAX) Jacobi steps.  uv is the
+// work-group's LDS window of (TW + 2*KMAX) x (TH + 2*KMAX) float2, stored with
+// the fixed row pitch TW + 2*KMAX so every tap is a constant LDS delta.
+// NTAP > 0: the stencil has exactly NTAP taps (straight-line code, batched LDS
+// reads); NTAP == 0: any count up to HS_MAX_TAPS, tested per tap.
+template <int NT, int TW, int TH, int KMAX, int NTAP>
+__device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
+                                        const HsTapTable<TW + 2 * KMAX> &tp, float2 *uv) {
+    constexpr int PITCH = TW + 2 * KMAX, RHM = TH + 2 * KMAX;
+    constexpr int NPX = PITCH * RHM;
+    constexpr int P = (NPX + NT - 1) / NT;
+    const int tid = threadIdx.x;
+    const int tyi = t / a.tiles_x, txi = t - tyi * a.tiles_x;
+    const int x0 = txi * TW, y0 = tyi * TH;
+    const int x1 = min(x0 + TW, a.W), y1 = min(y0 + TH, a.H);
+    const int gx0 = max(x0 - k, 0), gy0 = max(y0 - k, 0);
+    const int gx1 = min(x1 + k, a.W), gy1 = min(y1 + k, a.H);
+    const int RW = gx1 - gx0, RH = gy1 - gy0, npx = RW * RH;
+    const bool clipped = (x0 - k < 0) || (y0 - k < 0) || (x1 + k > a.W) || (y1 + k > a.H);
+
+    float cx[P], cy[P], ct[P], cd[P];
+    int pk[P];  // (ry << 16) | rx of each owned region pixel
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        const int i = tid + j * NT;
+        pk[j] = -1;
+        cx[j] = cy[j] = ct[j] = 0.0f;
+        cd[j] = 1.0f;
+        if (i < npx) {
+            const int ry = i / RW, rx = i - ry * RW;
+            pk[j] = (ry << 16) | rx;
+            const int64_t g = (int64_t)(gy0 + ry) * a.W + gx0 + rx;
+            const float ix = a.Ix[g], iy = a.Iy[g];
+            cx[j] = ix;
+            cy[j] = iy;
+            ct[j] = a.It[g];
+            if (a.den) {
+                cd[j] = a.den[g];
+            } else {
+                const float m1 = ix * ix;
+                const float s1 = a.alpha2 + m1;
+                const float m2 = iy * iy;
+                cd[j] = s1 + m2;
+            }
+            uv[ry * PITCH + rx] = make_float2(a.u_in[g], a.v_in[g]);
+        }
+    }
+    __syncthreads();
+
+    for (int it = 1; it <= k; ++it) {
+        const int e = k - it;
+        const int rx0 = max(x0 - e, 0) - gx0, ry0 = max(y0 - e, 0) - gy0;
+        const int rx1 = min(x1 + e, a.W) - gx0, ry1 = min(y1 + e, a.H) - gy0;
+        float nu[P], nv[P];
+        bool act[P];
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            // Opaque copy: stops the compiler from hoisting every pixel's tap
+            // addresses out of the iteration loop (P x taps live registers).
+            int pkj = pk[j];
+            asm volatile("" : "+v"(pkj));
+            const int rx = pkj & 0xffff, ry = pkj >> 16;
+            act[j] = pkj >= 0 && rx >= rx0 && rx < rx1 && ry >= ry0 && ry < ry1;
+            nu[j] = nv[j] = 0.0f;
+            if (act[j]) {
+                float ub = 0.0f, vb = 0.0f;
+                if (!clipped) {
+                    const float2 *base = uv + ry * PITCH + rx;
+#pragma unroll
+                    for (int d = 0; d < (NTAP ? NTAP : HS_MAX_TAPS); ++d) {
+                        if (NTAP || d < tp.nd) {
+                            const float2 q = base[tp.dl[d]];
+                            ub = fmaf(tp.w[d], q.x, ub);
+                            vb = fmaf(tp.w[d], q.y, vb);
+                        }
+                    }
+                } else {
+                    const int gx = gx0 + rx, gy = gy0 + ry;
+#pragma unroll
+                    for (int d = 0; d < (NTAP ? NTAP : HS_MAX_TAPS); ++d) {
+                        if (NTAP || d < tp.nd) {
+                            const int nx = clampi(gx + tp.ox[d], 0, a.W - 1) - gx0;
+                            const int ny = clampi(gy + tp.oy[d], 0, a.H - 1) - gy0;
+                            const float2 q = uv[ny * PITCH + nx];
+                            ub = fmaf(tp.w[d], q.x, ub);
+                            vb = fmaf(tp.w[d], q.y, vb);
+                        }
+                    }
+                }
+                const float m1 = cx[j] * ub;
+                const float m2 = cy[j] * vb;
+                const float s1 = m1 + m2;
+                const float s2 = s1 + ct[j];
+                const float tt = s2 / cd[j];
+                const float p1 = cx[j] * tt;
+                const float p2 = cy[j] * tt;
+                nu[j] = ub - p1;
+                nv[j] = vb - p2;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < P; ++j)
+            if (act[j]) uv[(pk[j] >> 16) * PITCH + (pk[j] & 0xffff)] = make_float2(nu[j], nv[j]);
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        if (pk[j] < 0) continue;
+        const int rx = pk[j] & 0xffff, ry = pk[j] >> 16;
+        const int gx = gx0 + rx, gy = gy0 + ry;
+        if (gx >= x0 && gx < x1 && gy >= y0 && gy < y1) {
+            const int64_t g = (int64_t)gy * a.W + gx;
+            const float2 q = uv[ry * PITCH + rx];
+            a.u_out[g] = q.x;
+            a.v_out[g] = q.y;
+        }
+    }
+    __syncthreads();  // the window is reused by the work-group's next tile
+}
+
+}  // namespace hmx

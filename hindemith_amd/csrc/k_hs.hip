@@ -19,7 +19,7 @@
 // Replicated borders (Stencil2DMVM.cpp:42-45 clamp): a tile whose grown
 // region was clipped by the image uses clamped LDS coordinates; all others use
 // precomputed tap deltas.
-#include "hmx_common.h"
+#include "hs_tile.h"
 
 #include <map>
 #include <mutex>
@@ -34,117 +34,31 @@ struct HsArgs {
     const float *Ix, *Iy, *It, *den;
     const float *u_in, *v_in;
     float *u_out, *v_out;
+    const float *alpha2_b;  // per-pair alpha^2 (device), or null: use alpha2
     float alpha2;
     int H, W, k, tiles_x, ntiles;
+    int64_t bstride;        // elements between consecutive pairs (blockIdx.y)
 };
 
-template <int NT, int TW, int TH, int KMAX>
+template <int NT, int TW, int TH, int KMAX, int NTAP>
 __global__ __launch_bounds__(NT) void hs_tb_kernel(HsArgs a, Taps taps) {
-    constexpr int RWM = TW + 2 * KMAX, RHM = TH + 2 * KMAX;
-    constexpr int NPX = RWM * RHM;
-    constexpr int P = (NPX + NT - 1) / NT;
-    __shared__ float2 uv[NPX];
-
-    const int tid = threadIdx.x;
-    const int t = xcd_remap(blockIdx.x, a.ntiles);
-    const int tyi = t / a.tiles_x, txi = t - tyi * a.tiles_x;
-    const int x0 = txi * TW, y0 = tyi * TH;
-    const int x1 = min(x0 + TW, a.W), y1 = min(y0 + TH, a.H);
-    const int k = a.k;
-    const int gx0 = max(x0 - k, 0), gy0 = max(y0 - k, 0);
-    const int gx1 = min(x1 + k, a.W), gy1 = min(y1 + k, a.H);
-    const int RW = gx1 - gx0, RH = gy1 - gy0, npx = RW * RH;
-    const bool clipped = (x0 - k < 0) || (y0 - k < 0) || (x1 + k > a.W) || (y1 + k > a.H);
-
-    float cx[P], cy[P], ct[P], cd[P];
-    int pk[P];  // (ry << 16) | rx of each owned region pixel
-#pragma unroll
-    for (int j = 0; j < P; ++j) {
-        const int i = tid + j * NT;
-        pk[j] = -1;
-        cx[j] = cy[j] = ct[j] = 0.0f;
-        cd[j] = 1.0f;
-        if (i < npx) {
-            const int ry = i / RW, rx = i - ry * RW;
-            pk[j] = (ry << 16) | rx;
-            const int64_t g = (int64_t)(gy0 + ry) * a.W + gx0 + rx;
-            const float ix = a.Ix[g], iy = a.Iy[g];
-            cx[j] = ix;
-            cy[j] = iy;
-            ct[j] = a.It[g];
-            if (a.den) {
-                cd[j] = a.den[g];
-            } else {
-                const float m1 = ix * ix;
-                const float s1 = a.alpha2 + m1;
-                const float m2 = iy * iy;
-                cd[j] = s1 + m2;
-            }
-            uv[i] = make_float2(a.u_in[g], a.v_in[g]);
-        }
-    }
-    __syncthreads();
-
-    for (int it = 1; it <= k; ++it) {
-        const int e = k - it;
-        const int rx0 = max(x0 - e, 0) - gx0, ry0 = max(y0 - e, 0) - gy0;
-        const int rx1 = min(x1 + e, a.W) - gx0, ry1 = min(y1 + e, a.H) - gy0;
-        float nu[P], nv[P];
-        bool act[P];
-#pragma unroll
-        for (int j = 0; j < P; ++j) {
-            const int rx = pk[j] & 0xffff, ry = pk[j] >> 16;
-            act[j] = pk[j] >= 0 && rx >= rx0 && rx < rx1 && ry >= ry0 && ry < ry1;
-            nu[j] = nv[j] = 0.0f;
-            if (act[j]) {
-                float ub = 0.0f, vb = 0.0f;
-                if (!clipped) {
-                    const int i = ry * RW + rx;
-                    for (int d = 0; d < taps.nd; ++d) {
-                        const float2 q = uv[i + taps.oy[d] * RW + taps.ox[d]];
-                        ub = fmaf(taps.w[d], q.x, ub);
-                        vb = fmaf(taps.w[d], q.y, vb);
-                    }
-                } else {
-                    const int gx = gx0 + rx, gy = gy0 + ry;
-                    for (int d = 0; d < taps.nd; ++d) {
-                        const int nx = clampi(gx + taps.ox[d], 0, a.W - 1) - gx0;
-                        const int ny = clampi(gy + taps.oy[d], 0, a.H - 1) - gy0;
-                        const float2 q = uv[ny * RW + nx];
-                        ub = fmaf(taps.w[d], q.x, ub);
-                        vb = fmaf(taps.w[d], q.y, vb);
-                    }
-                }
-                const float m1 = cx[j] * ub;
-                const float m2 = cy[j] * vb;
-                const float s1 = m1 + m2;
-                const float s2 = s1 + ct[j];
-                const float tt = s2 / cd[j];
-                const float p1 = cx[j] * tt;
-                const float p2 = cy[j] * tt;
-                nu[j] = ub - p1;
-                nv[j] = vb - p2;
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < P; ++j)
-            if (act[j]) uv[(pk[j] >> 16) * RW + (pk[j] & 0xffff)] = make_float2(nu[j], nv[j]);
-        __syncthreads();
-    }
-
-#pragma unroll
-    for (int j = 0; j < P; ++j) {
-        if (pk[j] < 0) continue;
-        const int rx = pk[j] & 0xffff, ry = pk[j] >> 16;
-        const int gx = gx0 + rx, gy = gy0 + ry;
-        if (gx >= x0 && gx < x1 && gy >= y0 && gy < y1) {
-            const int64_t g = (int64_t)gy * a.W + gx;
-            const float2 q = uv[ry * RW + rx];
-            a.u_out[g] = q.x;
-            a.v_out[g] = q.y;
-        }
-    }
+    __shared__ float2 uv[(TW + 2 * KMAX) * (TH + 2 * KMAX)];
+    const int64_t off = (int64_t)blockIdx.y * a.bstride;  // pair of a batch
+    HsImage im;
+    im.Ix = a.Ix + off;
+    im.Iy = a.Iy + off;
+    im.It = a.It + off;
+    im.den = a.den ? a.den + off : nullptr;
+    im.u_in = a.u_in + off;
+    im.v_in = a.v_in + off;
+    im.u_out = a.u_out + off;
+    im.v_out = a.v_out + off;
+    im.alpha2 = a.alpha2_b ? a.alpha2_b[blockIdx.y] : a.alpha2;
+    im.H = a.H;
+    im.W = a.W;
+    im.tiles_x = a.tiles_x;
+    const HsTapTable<TW + 2 * KMAX> tp(taps);
+    hs_tile<NT, TW, TH, KMAX, NTAP>(im, xcd_remap(blockIdx.x, a.ntiles), a.k, tp, uv);
 }
 
 // ---------------------------------------------------------------------------
@@ -152,10 +66,25 @@ __global__ __launch_bounds__(NT) void hs_tb_kernel(HsArgs a, Taps taps) {
 // 8*(TW+2K)(TH+2K) bytes.
 struct HsVariant {
     int nt, tw, th, kmax;
-    void (*kernel)(HsArgs, Taps);
+    void (*kernel[5])(HsArgs, Taps);  // by tap class: 4, 5, 8, 9 taps, any
 };
 
-#define HS_V(NT, TW, TH, K) {NT, TW, TH, K, hs_tb_kernel<NT, TW, TH, K>}
+// tap count -> kernel slot (straight-line kernels for the common stencils)
+static int tap_class(int nd) {
+    switch (nd) {
+        case 4: return 0;
+        case 5: return 1;
+        case 8: return 2;
+        case 9: return 3;
+        default: return 4;
+    }
+}
+
+#define HS_V(NT, TW, TH, K)                                                             \
+    {NT, TW, TH, K,                                                                     \
+     {hs_tb_kernel<NT, TW, TH, K, 4>, hs_tb_kernel<NT, TW, TH, K, 5>,                  \
+      hs_tb_kernel<NT, TW, TH, K, 8>, hs_tb_kernel<NT, TW, TH, K, 9>,                  \
+      hs_tb_kernel<NT, TW, TH, K, 0>}}
 static const HsVariant kVariants[] = {
     HS_V(256, 64, 16, 1),     // 0: one iteration per launch (plain sweep)
     HS_V(256, 64, 32, 4),     // 1
@@ -188,9 +117,11 @@ static void default_choice(int H, int W, int &variant, int &k) {
 int launch_hs_tb(int variant, const float *Ix, const float *Iy, const float *It,
                  const float *den, float alpha2, int H, int W, const Taps &taps, int k,
                  const float *u_in, const float *v_in, float *u_out, float *v_out,
-                 hipStream_t s) {
+                 hipStream_t s, int B = 1, const float *alpha2_b = nullptr) {
     const HsVariant &V = kVariants[variant];
     HsArgs a;
+    a.alpha2_b = alpha2_b;
+    a.bstride = (int64_t)H * W;
     a.Ix = Ix;
     a.Iy = Iy;
     a.It = It;
@@ -206,15 +137,17 @@ int launch_hs_tb(int variant, const float *Ix, const float *Iy, const float *It,
     a.tiles_x = (W + V.tw - 1) / V.tw;
     const int tiles_y = (H + V.th - 1) / V.th;
     a.ntiles = a.tiles_x * tiles_y;
-    hipLaunchKernelGGL(V.kernel, dim3(a.ntiles), dim3(V.nt), 0, s, a, taps);
+    hipLaunchKernelGGL(V.kernel[tap_class(taps.nd)], dim3(a.ntiles, B), dim3(V.nt), 0, s, a,
+                       taps);
     HMX_LAUNCH_CHECK("hs_tb");
     return HMX_OK;
 }
 
-static int check_avg_taps(const Taps &t) {
-    if (t.rx > 1 || t.ry > 1)
+int check_avg_taps(const Taps &t) {
+    if (t.rx > 1 || t.ry > 1 || t.nd > HS_MAX_TAPS)
         return fail(HMX_ERR_UNSUPPORTED,
-                    "fused HS Jacobi needs an averaging stencil within radius 1");
+                    "fused HS Jacobi needs an averaging stencil within radius 1 and at most " +
+                        std::to_string(HS_MAX_TAPS) + " taps");
     return HMX_OK;
 }
 
@@ -222,26 +155,33 @@ static int check_avg_taps(const Taps &t) {
 // in (u,v).
 int hs_jacobi_run(const float *Ix, const float *Iy, const float *It, const float *den,
                   float alpha2, int H, int W, const Taps &taps, int iters, float *u, float *v,
-                  float *wu, float *wv, int variant, int k, hipStream_t s) {
+                  float *wu, float *wv, int variant, int k, hipStream_t s, int B,
+                  const float *alpha2_b) {
     if (iters <= 0) return HMX_OK;
     float *src_u = u, *src_v = v, *dst_u = wu, *dst_v = wv;
     int done = 0;
     while (done < iters) {
         const int kk = (iters - done) < k ? (iters - done) : k;
         int rc = launch_hs_tb(variant, Ix, Iy, It, den, alpha2, H, W, taps, kk, src_u, src_v,
-                              dst_u, dst_v, s);
+                              dst_u, dst_v, s, B, alpha2_b);
         if (rc) return rc;
         std::swap(src_u, dst_u);
         std::swap(src_v, dst_v);
         done += kk;
     }
     if (src_u != u) {
-        HMX_HIP_TRY(hipMemcpyAsync(u, src_u, sizeof(float) * (size_t)H * W,
-                                   hipMemcpyDeviceToDevice, s));
-        HMX_HIP_TRY(hipMemcpyAsync(v, src_v, sizeof(float) * (size_t)H * W,
-                                   hipMemcpyDeviceToDevice, s));
+        const size_t bytes = sizeof(float) * (size_t)H * W * B;
+        HMX_HIP_TRY(hipMemcpyAsync(u, src_u, bytes, hipMemcpyDeviceToDevice, s));
+        HMX_HIP_TRY(hipMemcpyAsync(v, src_v, bytes, hipMemcpyDeviceToDevice, s));
     }
     return HMX_OK;
+}
+
+int hs_jacobi_run(const float *Ix, const float *Iy, const float *It, const float *den,
+                  float alpha2, int H, int W, const Taps &taps, int iters, float *u, float *v,
+                  float *wu, float *wv, int variant, int k, hipStream_t s) {
+    return hs_jacobi_run(Ix, Iy, It, den, alpha2, H, W, taps, iters, u, v, wu, wv, variant, k,
+                         s, 1, nullptr);
 }
 
 void hs_choice(int H, int W, int &variant, int &k) {

@@ -1088,7 +1088,7 @@ static int exec_node(Record &R, Node &n) {
             Taps t;
             int rc = host_taps(R.syms[n.syms[6]], t);
             if (rc) return rc;
-            if (t.rx > 1 || t.ry > 1) {  // wider than the fused kernel handles
+            if (t.rx > 1 || t.ry > 1 || t.nd > 9) {  // beyond what the fused kernel takes
                 for (int i = 0; i < n.ub && !rc; ++i) rc = exec_nodes(R, n.body);
                 return rc;
             }

@@ -200,6 +200,51 @@ int hmx_hs_solve(const float *I1, const float *I2, int H, int W, float alpha2,
                  int nda, const float *wa, const int *oxa, const int *oya,
                  int iters, float *u, float *v, float *workspace, void *stream);
 
+/* ---------------------------------------------------------------------- */
+/* a13. Adjoint (no counterpart in the reference, which has no gradients).  */
+/* B equally sized pairs are stacked as B x H x W planes; alpha is a device  */
+/* array of B floats (den = alpha^2 + Ix^2 + Iy^2 per pair).                 */
+
+/* Number of checkpoints ceil(iters / ckpt_every). */
+int hmx_hs_ckpt_count(int iters, int ckpt_every);
+/* Workspace bytes for hmx_hs_forward_ckpt / hmx_hs_adjoint. */
+int64_t hmx_hs_forward_workspace(int B, int H, int W);
+int64_t hmx_hs_adjoint_workspace(int B, int H, int W, int ckpt_every);
+
+/* Forward solve that stores the state (u, v) every ckpt_every iterations:
+ * ckpt holds hmx_hs_ckpt_count() x 2 x B x H x W floats (state before
+ * iterations 0, C, 2C, ...).  u, v: initial state in, final state out. */
+int hmx_hs_forward_ckpt(const float *I1, const float *I2, int B, int H, int W,
+                        const float *alpha,
+                        int ndx, const float *wx, const int *oxx, const int *oyx,
+                        int ndy, const float *wy, const int *oxy, const int *oyy,
+                        int ndt, const float *wt, const int *oxt, const int *oyt,
+                        int nda, const float *wa, const int *oxa, const int *oya,
+                        int iters, int ckpt_every, float *u, float *v, float *ckpt,
+                        void *workspace, void *stream);
+
+/* Reverse pass: given dL/du_N, dL/dv_N, writes dL/dI1, dL/dI2 (B x H x W),
+ * dL/dalpha (B, device) and, if non-null, dL/du_0, dL/dv_0.  Recomputes one
+ * checkpoint segment at a time; averaging stencil within radius 1. */
+int hmx_hs_adjoint(const float *I1, const float *I2, int B, int H, int W, const float *alpha,
+                   int ndx, const float *wx, const int *oxx, const int *oyx,
+                   int ndy, const float *wy, const int *oxy, const int *oyy,
+                   int ndt, const float *wt, const int *oxt, const int *oyt,
+                   int nda, const float *wa, const int *oxa, const int *oya,
+                   int iters, int ckpt_every, const float *ckpt, const float *gu_N,
+                   const float *gv_N, float *gI1, float *gI2, float *galpha, float *gu0,
+                   float *gv0, void *workspace, void *stream);
+
+/* fp64 forward solve (plain per-iteration kernels) of B pairs, for
+ * finite-difference checks.  alpha: device doubles; workspace 5*B*H*W doubles. */
+int hmx_hs_solve_f64(const double *I1, const double *I2, int B, int H, int W,
+                     const double *alpha,
+                     int ndx, const float *wx, const int *oxx, const int *oyx,
+                     int ndy, const float *wy, const int *oxy, const int *oyy,
+                     int ndt, const float *wt, const int *oxt, const int *oyt,
+                     int nda, const float *wa, const int *oxa, const int *oya,
+                     int iters, double *u, double *v, double *workspace, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -186,3 +186,35 @@ def test_hs_tune_then_solve(K):
     u, v = K.hs_solve(dev(I1), dev(I2), 1.0, TAPS, 37)
     ru, rv = oracle.hs_solve(I1, I2, 1.0, TAPS, 37)
     assert np.array_equal(host(u), ru) and np.array_equal(host(v), rv)
+
+
+def _avg_stencil(kind):
+    e, c = 1.0 / 6.0, 1.0 / 12.0
+    if kind == 4:
+        return (np.full(4, 0.25, np.float32), np.array([-1, 1, 0, 0], np.int32),
+                np.array([0, 0, -1, 1], np.int32))
+    if kind == 5:
+        return (np.array([0.2] * 5, np.float32), np.array([0, -1, 1, 0, 0], np.int32),
+                np.array([0, 0, 0, -1, 1], np.int32))
+    if kind == 9:
+        return (np.full(9, 1.0 / 9.0, np.float32), np.array([-1, 0, 1] * 3, np.int32),
+                np.repeat(np.array([-1, 0, 1], np.int32), 3))
+    if kind == 7:  # runtime tap-count path
+        return (np.array([e, e, e, e, c, c, 2 * c], np.float32),
+                np.array([-1, 1, 0, 0, -1, 1, 0], np.int32),
+                np.array([0, 0, -1, 1, -1, 1, 0], np.int32))
+    return TAPS["Avg"]
+
+
+@pytest.mark.parametrize("kind", [4, 5, 7, 8, 9])
+def test_hs_jacobi_tap_classes_bitexact(K, kind):
+    shape = (77, 150)
+    I1, I2, _, _ = S.hs_pair(*shape, seed=8)
+    Ix, Iy, It = oracle.hs_setup(I1, I2, TAPS["Dx"], TAPS["Dy"], TAPS["Dt"])
+    avg = _avg_stencil(kind)
+    ref_u, ref_v = oracle.hs_jacobi(Ix, Iy, It, 0.8, avg, 11)
+    for vi, k in _variants(K):
+        u = torch.zeros(shape, device="cuda")
+        v = torch.zeros(shape, device="cuda")
+        K.hs_jacobi(dev(Ix), dev(Iy), dev(It), 0.8, avg, 11, u, v, variant=vi, k=k)
+        assert np.array_equal(host(u), ref_u) and np.array_equal(host(v), ref_v), (kind, vi, k)

@@ -42,7 +42,24 @@ def parse():
     p.add_argument("--no-tune", action="store_true", help="use the default kernel choice")
     p.add_argument("--variant", type=int, default=None)
     p.add_argument("--k", type=int, default=None)
+    p.add_argument("--no-extras", action="store_true",
+                   help="skip the secondary workloads (4K, bandwidth kernels, batched, adjoint)")
+    p.add_argument("--copy-mb", type=int, default=0,
+                   help="only run a plain device copy of this many MiB (counter calibration)")
     return p.parse_args()
+
+
+def copy_calibration(mb, reps=5):
+    """A plain device-to-device copy of known size, for scaling rocprofv3's
+    FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    n = mb * (1 << 20) // 4
+    x = torch.rand(n, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "reps": reps}))
 
 
 def cpu_baseline(I1, I2, taps, H, W):
@@ -63,6 +80,147 @@ def cpu_baseline(I1, I2, taps, H, W):
                       f"median of 5 runs after 1 warm-up ({med:.2f} s each)"}
 
 
+def _time_ms(fn, reps, warm=2):
+    import torch
+    for _ in range(warm):
+        fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def extra_4k(K, S, taps):
+    """3840x2160 fp32, 500 iterations, affine flow (same kernel path)."""
+    import torch
+    H, W, iters = 2160, 3840, 500
+    I1h, I2h, _, _ = S.hs_pair(H, W, seed=1, flow="affine")
+    I1, I2 = torch.from_numpy(I1h).cuda(), torch.from_numpy(I2h).cuda()
+    ws = torch.empty((5, H, W), device="cuda")
+    u = torch.zeros((H, W), device="cuda")
+    v = torch.zeros_like(u)
+    Ix, Iy, It = K.hs_setup(I1, I2, taps["Dx"], taps["Dy"], taps["Dt"], out=ws[0:3])
+    vi, k = K.hs_tune(Ix, Iy, It, 1.0, taps["Avg"])
+
+    def solve():
+        u.zero_()
+        v.zero_()
+        K.hs_solve(I1, I2, 1.0, taps, iters, u, v, workspace=ws)
+    ms = _time_ms(solve, 3, warm=1)
+    val = H * W * iters / (ms * 1e-3) / 1e6
+    return {"workload": f"{W}x{H} fp32 {iters} it", "value": round(val, 1),
+            "unit": "Mpixel*iter/s", "ms_per_solve": round(ms, 3), "variant": vi, "k": k,
+            "hbm_roofline_fraction": round(val / (HBM_PEAK_GBS * 1e3 / BYTES_PER_PX_ITER), 4)}
+
+
+def extra_bandwidth(K, S):
+    """2048x2048 bandwidth kernels, 200 repeats each (GB/s of algorithmic bytes)."""
+    import torch
+    H = W = 2048
+    N = H * W
+    rng = np.random.default_rng(0)
+    x = torch.from_numpy(rng.uniform(-1, 1, (H, W)).astype(np.float32)).cuda()
+    y = torch.empty_like(x)
+    out = {}
+    lap = S.laplacian5_taps()
+    ms = _time_ms(lambda: K.stencil2d(x, lap, out=y), 200)
+    out["stencil_5pt"] = {"us": round(ms * 1e3, 2), "GBps": round(8 * N / ms / 1e6, 1)}
+    planes, ox, oy = S.laplacian5_dia_planes(H, W)
+    A = torch.from_numpy(planes).cuda()
+    ms = _time_ms(lambda: K.dia2d_spmv(A, ox, oy, x, out=y), 200)
+    out["dia2d_5diag"] = {"us": round(ms * 1e3, 2), "GBps": round(28 * N / ms / 1e6, 1)}
+    ip, ix, d = S.dia2d_to_csr(planes, ox, oy)
+    tip, tix, td = (torch.from_numpy(a).cuda() for a in (ip, ix, d))
+    xf = x.reshape(-1)
+    yf = y.reshape(-1)
+    nnz = int(ip[-1])
+    ms = _time_ms(lambda: K.csr_spmv(tip, tix, td, xf, out=yf), 200)
+    out["csr_5pt"] = {"nnz": nnz, "us": round(ms * 1e3, 2),
+                      "GBps": round((8 * nnz + 12 * N) / ms / 1e6, 1)}
+    del tip, tix, td
+    ip, ix, d = S.powerlaw_csr(N, np.random.default_rng(1))
+    tip, tix, td = (torch.from_numpy(a).cuda() for a in (ip, ix, d))
+    nnz = int(ip[-1])
+    bins = K.CSRBins(ip, "cuda")
+    ms = _time_ms(lambda: K.csr_spmv_binned(tip, tix, td, xf, bins, out=yf), 200)
+    ms_vec = _time_ms(lambda: K.csr_spmv(tip, tix, td, xf, out=yf), 50)
+    out["csr_powerlaw_zipf1.5"] = {"nnz": nnz, "us": round(ms * 1e3, 2),
+                                   "GBps": round((8 * nnz + 12 * N) / ms / 1e6, 1),
+                                   "us_unbinned": round(ms_vec * 1e3, 2)}
+    return out
+
+
+def extra_batched(S):
+    """64 uneven pairs (sides U(64,1024)), 200 iterations: one persistent
+    graph-captured launch vs 64 plain per-pair solves."""
+    import torch
+    from hindemith_amd.batched import BatchedHS
+    pairs = S.batched_pairs(64, seed=0)
+    B = BatchedHS([torch.from_numpy(p["I1"]).cuda() for p in pairs],
+                  [torch.from_numpy(p["I2"]).cuda() for p in pairs],
+                  [p["alpha"] for p in pairs], iters=200)
+    vi, k = B.tune()
+    B.run(graph=True, sync=True)
+    ms_b = _time_ms(lambda: B.run(graph=True), 5)
+    ms_p = _time_ms(B.solve_each, 3, warm=1)
+    px = sum(p["I1"].size for p in pairs)
+    return {"pairs": 64, "megapixels": round(px / 1e6, 2), "iters": 200,
+            "batched_pairs_per_s": round(64 / (ms_b * 1e-3), 1),
+            "plain_pairs_per_s": round(64 / (ms_p * 1e-3), 1),
+            "speedup": round(ms_p / ms_b, 2), "batched_ms": round(ms_b, 3),
+            "plain_ms": round(ms_p, 3), "variant": vi, "k": k}
+
+
+def extra_adjoint(S):
+    """8 pairs at 512x512, 200 iterations, checkpoints every 20: time of the
+    hand-written backward vs the forward."""
+    import torch
+    from hindemith_amd.autograd import horn_schunck
+    rng = np.random.default_rng(0)
+    I1, I2, fx, fy = [], [], [], []
+    for b in range(8):
+        a, c, x, y = S.hs_pair(512, 512, seed=100 + b, flow="translation",
+                               trans=tuple(rng.uniform(-3, 3, 2)))
+        I1.append(a), I2.append(c), fx.append(x), fy.append(y)
+    tI1 = torch.tensor(np.stack(I1), device="cuda", requires_grad=True)
+    tI2 = torch.tensor(np.stack(I2), device="cuda", requires_grad=True)
+    al = torch.tensor(rng.uniform(0.5, 2, 8).astype(np.float32), device="cuda",
+                      requires_grad=True)
+    tfx = torch.tensor(np.stack(fx), device="cuda")
+    tfy = torch.tensor(np.stack(fy), device="cuda")
+    st = {}
+
+    def fwd():
+        st["u"], st["v"] = horn_schunck(tI1, tI2, al, iters=200, ckpt_every=20)
+
+    def bwd():
+        loss = ((st["u"] - tfx) ** 2 + (st["v"] - tfy) ** 2).mean()
+        loss.backward()
+
+    fwd()
+    bwd()
+    torch.cuda.synchronize()
+    t_f, t_b = [], []
+    for _ in range(3):
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        e[0].record()
+        fwd()
+        e[1].record()
+        bwd()
+        e[2].record()
+        torch.cuda.synchronize()
+        t_f.append(e[0].elapsed_time(e[1]))
+        t_b.append(e[1].elapsed_time(e[2]))
+    f, b = float(np.median(t_f)), float(np.median(t_b))
+    return {"workload": "8 x 512x512, 200 it, ckpt every 20", "forward_ms": round(f, 3),
+            "backward_ms": round(b, 3), "backward_over_forward": round(b / f, 2)}
+
+
 def load_traffic(variant, k, H, W):
     """HBM bytes per launch from the committed rocprofv3 PMC summary, if it
     was measured for this exact configuration."""
@@ -78,6 +236,9 @@ def load_traffic(variant, k, H, W):
 
 def main():
     args = parse()
+    if args.copy_mb:
+        copy_calibration(args.copy_mb)
+        return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (the "
                                    "multi-GPU design is domain decomposition, not built here)"}))
@@ -171,6 +332,18 @@ def main():
                    "endpoint_error_px": round(epe, 4)},
         "roofline": roofline,
     }
+    if not args.no_extras:
+        sec = {}
+        for name, fn in (("hs_3840x2160", lambda: extra_4k(K, S, taps)),
+                         ("bandwidth_2048x2048", lambda: extra_bandwidth(K, S)),
+                         ("batched_vs_plain", lambda: extra_batched(S)),
+                         ("adjoint_vs_forward", lambda: extra_adjoint(S))):
+            try:
+                sec[name] = fn()
+            except Exception as e:  # a secondary workload must not hide the headline
+                sec[name] = {"error": f"{type(e).__name__}: {e}"}
+            print(f"[bench] {name}: {json.dumps(sec[name])}", file=sys.stderr, flush=True)
+        out["secondary"] = sec
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(I1h, I2h, taps, H, W)
     print(json.dumps(out))

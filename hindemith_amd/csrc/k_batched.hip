@@ -1,0 +1,468 @@
+// a14: many Horn-Schunck pairs of uneven sizes in one persistent launch
+// (no counterpart in the reference, which runs one program call at a time).
+//
+// The tiles of all pairs form one list.  A grid no larger than what the GPU
+// holds at once walks that list: first the derivative setup of every tile,
+// then ceil(iters / k) phases of k fused Jacobi steps (the same tile body as
+// the single-image kernel, hs_tile.h), with a grid-wide barrier between
+// phases so every tile of every pair sees the previous phase's halo.  Pairs
+// ping-pong between (u, v) and (wu, wv); an odd phase count ends with a copy.
+//
+// The barrier is an arrival counter in device memory: each work-group's
+// first thread releases its writes (agent-scope fence), increments the
+// counter and waits until all work-groups of the grid have arrived, then
+// acquires.  The wait is bounded: after spin_limit polls, or as soon as any
+// work-group has given up, it sets *status and every work-group returns, so
+// the grid always drains.  The launch is captured in a hipGraph (counter and
+// status reset by a fill kernel, never a memset node) and replayed.
+#include "hs_tile.h"
+
+#include <algorithm>
+#include <cstring>
+#include <list>
+#include <mutex>
+#include <vector>
+
+#pragma clang fp contract(off)
+
+namespace hmx {
+
+struct PairDesc {
+    const float *I1, *I2;
+    float *Ix, *Iy, *It;
+    float *u, *v, *wu, *wv;
+    float alpha2;
+    int H, W, tiles_x, tile_base, ntiles;
+};
+
+struct BatchArgs {
+    const PairDesc *pairs;
+    unsigned *barrier;
+    int *status;
+    long long spin_limit;
+    int npairs, total_tiles, iters, k;
+};
+
+struct SetupTaps {
+    Taps dx, dy, dt;
+};
+
+__device__ inline int find_pair(const PairDesc *P, int n, int w) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (P[mid].tile_base <= w) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ bool grid_sync(unsigned *ctr, unsigned target, int *status,
+                                          long long limit) {
+    __shared__ int ok;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        atomicAdd(ctr, 1u);
+        int good = 1;
+        long long n = 0;
+        while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+                ++n > limit) {
+                good = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        if (!good) atomicExch(status, 1);
+        __threadfence();
+        ok = good;
+    }
+    __syncthreads();
+    return ok != 0;
+}
+
+__device__ inline float setup_tap(const float *__restrict__ x, int H, int W, int r, int c,
+                                  const Taps &t) {
+    float acc = 0.0f;
+    for (int d = 0; d < t.nd; ++d) {
+        const int xx = clampi(c + t.ox[d], 0, W - 1), yy = clampi(r + t.oy[d], 0, H - 1);
+        acc = fmaf(t.w[d], x[(int64_t)yy * W + xx], acc);
+    }
+    return acc;
+}
+
+template <int NT, int TW, int TH, int KMAX, int NTAP>
+__global__ __launch_bounds__(NT) void hs_batched_kernel(BatchArgs b, SetupTaps st, Taps taps) {
+    __shared__ float2 uv[(TW + 2 * KMAX) * (TH + 2 * KMAX)];
+    const HsTapTable<TW + 2 * KMAX> tp(taps);
+    unsigned arrivals = 0;
+
+    // derivative setup of every tile (Ix = Dx*I1 + Dx*I2, ...), one rounding per op
+    for (int w = blockIdx.x; w < b.total_tiles; w += gridDim.x) {
+        const PairDesc &P = b.pairs[find_pair(b.pairs, b.npairs, w)];
+        const int t = w - P.tile_base;
+        const int tyi = t / P.tiles_x, txi = t - tyi * P.tiles_x;
+        for (int i = threadIdx.x; i < TW * TH; i += NT) {
+            const int r = tyi * TH + i / TW, c = txi * TW + i % TW;
+            if (r >= P.H || c >= P.W) continue;
+            const int64_t p = (int64_t)r * P.W + c;
+            const float a1 = setup_tap(P.I1, P.H, P.W, r, c, st.dx);
+            const float a2 = setup_tap(P.I2, P.H, P.W, r, c, st.dx);
+            P.Ix[p] = a1 + a2;
+            const float b1 = setup_tap(P.I1, P.H, P.W, r, c, st.dy);
+            const float b2 = setup_tap(P.I2, P.H, P.W, r, c, st.dy);
+            P.Iy[p] = b1 + b2;
+            const float t2 = setup_tap(P.I2, P.H, P.W, r, c, st.dt);
+            const float t1 = setup_tap(P.I1, P.H, P.W, r, c, st.dt);
+            P.It[p] = t2 - t1;
+        }
+    }
+    arrivals += gridDim.x;
+    if (!grid_sync(b.barrier, arrivals, b.status, b.spin_limit)) return;
+
+    const int nph = (b.iters + b.k - 1) / b.k;
+    for (int ph = 0; ph < nph; ++ph) {
+        const int kk = min(b.k, b.iters - ph * b.k);
+        for (int w = blockIdx.x; w < b.total_tiles; w += gridDim.x) {
+            const PairDesc &P = b.pairs[find_pair(b.pairs, b.npairs, w)];
+            HsImage im;
+            im.Ix = P.Ix;
+            im.Iy = P.Iy;
+            im.It = P.It;
+            im.den = nullptr;
+            const bool even = (ph & 1) == 0;
+            im.u_in = even ? P.u : P.wu;
+            im.v_in = even ? P.v : P.wv;
+            im.u_out = even ? P.wu : P.u;
+            im.v_out = even ? P.wv : P.v;
+            im.alpha2 = P.alpha2;
+            im.H = P.H;
+            im.W = P.W;
+            im.tiles_x = P.tiles_x;
+            hs_tile<NT, TW, TH, KMAX, NTAP>(im, w - P.tile_base, kk, tp, uv);
+        }
+        arrivals += gridDim.x;
+        if (!grid_sync(b.barrier, arrivals, b.status, b.spin_limit)) return;
+    }
+
+    if (nph & 1) {  // the last phase wrote (wu, wv)
+        for (int w = blockIdx.x; w < b.total_tiles; w += gridDim.x) {
+            const PairDesc &P = b.pairs[find_pair(b.pairs, b.npairs, w)];
+            const int t = w - P.tile_base;
+            const int tyi = t / P.tiles_x, txi = t - tyi * P.tiles_x;
+            for (int i = threadIdx.x; i < TW * TH; i += NT) {
+                const int r = tyi * TH + i / TW, c = txi * TW + i % TW;
+                if (r >= P.H || c >= P.W) continue;
+                const int64_t p = (int64_t)r * P.W + c;
+                P.u[p] = P.wu[p];
+                P.v[p] = P.wv[p];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+
+struct BVariant {
+    int nt, tw, th, kmax;
+    void (*kernel[2])(BatchArgs, SetupTaps, Taps);  // 8 taps, any count
+};
+
+#define HB_V(NT, TW, TH, K)                                                               \
+    {NT, TW, TH, K, {hs_batched_kernel<NT, TW, TH, K, 8>, hs_batched_kernel<NT, TW, TH, K, 0>}}
+static const BVariant kBVariants[] = {
+    HB_V(1024, 64, 32, 4),   // 0
+    HB_V(1024, 64, 32, 8),   // 1
+    HB_V(1024, 128, 32, 8),  // 2
+    HB_V(512, 64, 32, 4),    // 3
+    HB_V(1024, 64, 64, 8),   // 4
+};
+#undef HB_V
+static constexpr int kNumBVariants = sizeof(kBVariants) / sizeof(kBVariants[0]);
+
+// A cached launch: device descriptors, barrier words and (optionally) the
+// captured graph, keyed by everything the launch bakes in.
+struct BatchPlan {
+    std::vector<char> key;
+    int device = 0;
+    int variant = 0, k = 1, grid = 0, total_tiles = 0;
+    PairDesc *d_pairs = nullptr;
+    unsigned *d_sync = nullptr;  // [0] barrier counter, [1] status
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+static std::mutex g_bmu;
+static std::list<BatchPlan> g_plans;  // most recent first
+static constexpr size_t kMaxPlans = 8;
+static std::vector<std::pair<std::vector<int>, std::pair<int, int>>> g_btuned;
+
+static void free_plan(BatchPlan &p) {
+    if (p.exec) (void)hipGraphExecDestroy(p.exec);
+    if (p.graph) (void)hipGraphDestroy(p.graph);
+    if (p.d_pairs) (void)hipFree(p.d_pairs);
+    if (p.d_sync) (void)hipFree(p.d_sync);
+    p = BatchPlan();
+}
+
+template <typename T>
+static void append(std::vector<char> &k, const T &v) {
+    const char *c = reinterpret_cast<const char *>(&v);
+    k.insert(k.end(), c, c + sizeof(T));
+}
+
+static int launch_batch(const BatchPlan &P, const SetupTaps &st, const Taps &av, int npairs,
+                        int iters, long long spin_limit, hipStream_t s) {
+    int rc = launch_fill_u32(P.d_sync, 0u, 2, s);
+    if (rc) return rc;
+    BatchArgs a;
+    a.pairs = P.d_pairs;
+    a.barrier = P.d_sync;
+    a.status = (int *)(P.d_sync + 1);
+    a.spin_limit = spin_limit;
+    a.npairs = npairs;
+    a.total_tiles = P.total_tiles;
+    a.iters = iters;
+    a.k = P.k;
+    const BVariant &V = kBVariants[P.variant];
+    hipLaunchKernelGGL(V.kernel[av.nd == 8 ? 0 : 1], dim3(P.grid), dim3(V.nt), 0, s, a, st, av);
+    HMX_LAUNCH_CHECK("hs_batched");
+    return HMX_OK;
+}
+
+static int resident_grid(int variant, int nd, int total_tiles, int device, int &grid) {
+    const BVariant &V = kBVariants[variant];
+    int per_cu = 0, cus = 0;
+    HMX_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void *>(V.kernel[nd == 8 ? 0 : 1]), V.nt, 0));
+    HMX_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    if (per_cu < 1) return fail(HMX_ERR_UNSUPPORTED, "batched kernel cannot be resident");
+    grid = std::min(total_tiles, per_cu * cus);
+    return HMX_OK;
+}
+
+}  // namespace hmx
+
+using namespace hmx;
+
+extern "C" int hmx_hs_batched_variant_count(void) { return kNumBVariants; }
+
+extern "C" int hmx_hs_batched_variant_info(int variant, int *tile_w, int *tile_h, int *threads,
+                                           int *kmax) {
+    HMX_REQUIRE(variant >= 0 && variant < kNumBVariants, "unknown batched variant");
+    const BVariant &V = kBVariants[variant];
+    if (tile_w) *tile_w = V.tw;
+    if (tile_h) *tile_h = V.th;
+    if (threads) *threads = V.nt;
+    if (kmax) *kmax = V.kmax;
+    return HMX_OK;
+}
+
+static int batched_impl(const hmx_pair *pairs, int npairs, const SetupTaps &st, const Taps &av,
+                        int iters, int variant, int k, int flags, long long spin_limit,
+                        hipStream_t s) {
+    HMX_REQUIRE(pairs && npairs >= 1, "hmx_hs_batched: no pairs");
+    HMX_REQUIRE(iters >= 0, "hmx_hs_batched: iters must be >= 0");
+    HMX_REQUIRE(av.rx <= 1 && av.ry <= 1 && av.nd <= HS_MAX_TAPS,
+                "hmx_hs_batched: averaging stencil must lie within radius 1 with <= 9 taps");
+    if (variant < 0) {  // autotuned choice for this size signature, else default
+        variant = 1;
+        k = 8;
+        std::vector<int> sig;
+        for (int i = 0; i < npairs; ++i) {
+            sig.push_back(pairs[i].H);
+            sig.push_back(pairs[i].W);
+        }
+        std::lock_guard<std::mutex> lk(g_bmu);
+        for (auto &e : g_btuned)
+            if (e.first == sig) {
+                variant = e.second.first;
+                k = e.second.second;
+            }
+    }
+    HMX_REQUIRE(variant < kNumBVariants, "hmx_hs_batched: unknown variant");
+    HMX_REQUIRE(k >= 1 && k <= kBVariants[variant].kmax, "hmx_hs_batched: k out of range");
+    const BVariant &V = kBVariants[variant];
+    std::vector<PairDesc> desc(npairs);
+    int tiles = 0;
+    for (int i = 0; i < npairs; ++i) {
+        const hmx_pair &p = pairs[i];
+        HMX_REQUIRE(p.I1 && p.I2 && p.u && p.v && p.work, "hmx_hs_batched: null pair buffer");
+        HMX_REQUIRE(p.H > 0 && p.W > 0 && p.H < 32768 && p.W < 32768,
+                    "hmx_hs_batched: pair sizes must be in 1..32767");
+        const int64_t n = (int64_t)p.H * p.W;
+        PairDesc &d = desc[i];
+        d.I1 = p.I1;
+        d.I2 = p.I2;
+        d.Ix = p.work;
+        d.Iy = p.work + n;
+        d.It = p.work + 2 * n;
+        d.wu = p.work + 3 * n;
+        d.wv = p.work + 4 * n;
+        d.u = p.u;
+        d.v = p.v;
+        d.alpha2 = p.alpha * p.alpha;
+        d.H = p.H;
+        d.W = p.W;
+        d.tiles_x = (p.W + V.tw - 1) / V.tw;
+        d.ntiles = d.tiles_x * ((p.H + V.th - 1) / V.th);
+        d.tile_base = tiles;
+        HMX_REQUIRE((int64_t)tiles + d.ntiles < (1 << 30), "hmx_hs_batched: too many tiles");
+        tiles += d.ntiles;
+    }
+    int device = 0;
+    HMX_HIP_TRY(hipGetDevice(&device));
+    std::vector<char> key;
+    append(key, device);
+    append(key, variant);
+    append(key, k);
+    append(key, iters);
+    append(key, spin_limit);
+    append(key, st);
+    append(key, av);
+    append(key, (void *)s);
+    key.insert(key.end(), (const char *)desc.data(),
+               (const char *)desc.data() + sizeof(PairDesc) * desc.size());
+
+    std::lock_guard<std::mutex> lk(g_bmu);
+    BatchPlan *plan = nullptr;
+    for (auto it = g_plans.begin(); it != g_plans.end(); ++it)
+        if (it->key == key) {
+            g_plans.splice(g_plans.begin(), g_plans, it);
+            plan = &g_plans.front();
+            break;
+        }
+    if (!plan) {
+        BatchPlan P;
+        P.key = key;
+        P.device = device;
+        P.variant = variant;
+        P.k = k;
+        P.total_tiles = tiles;
+        int rc = resident_grid(variant, av.nd, tiles, device, P.grid);
+        if (rc) return rc;
+        HMX_HIP_TRY(hipMalloc(&P.d_pairs, sizeof(PairDesc) * npairs));
+        HMX_HIP_TRY(hipMalloc(&P.d_sync, 256));
+        HMX_HIP_TRY(hipMemcpy(P.d_pairs, desc.data(), sizeof(PairDesc) * npairs,
+                              hipMemcpyHostToDevice));
+        if (flags & HMX_BATCH_GRAPH) {
+            hipStream_t cs;
+            HMX_HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+            HMX_HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+            rc = launch_batch(P, st, av, npairs, iters, spin_limit, cs);
+            hipError_t e = hipStreamEndCapture(cs, &P.graph);
+            (void)hipStreamDestroy(cs);
+            if (rc) return rc;
+            if (e != hipSuccess) return fail(HMX_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
+            HMX_HIP_TRY(hipGraphInstantiate(&P.exec, P.graph, nullptr, nullptr, 0));
+        }
+        if (g_plans.size() >= kMaxPlans) {
+            free_plan(g_plans.back());
+            g_plans.pop_back();
+        }
+        g_plans.push_front(std::move(P));
+        plan = &g_plans.front();
+    }
+    if (iters == 0) return HMX_OK;
+    if ((flags & HMX_BATCH_GRAPH) && plan->exec) {
+        HMX_HIP_TRY(hipGraphLaunch(plan->exec, s));
+    } else {
+        int rc = launch_batch(*plan, st, av, npairs, iters, spin_limit, s);
+        if (rc) return rc;
+    }
+    if (flags & HMX_BATCH_SYNC) {
+        unsigned status = 0;
+        HMX_HIP_TRY(hipStreamSynchronize(s));
+        HMX_HIP_TRY(hipMemcpy(&status, plan->d_sync + 1, 4, hipMemcpyDeviceToHost));
+        if (status) return fail(HMX_ERR_TIMEOUT, "hmx_hs_batched: a grid barrier wait gave up");
+    }
+    return HMX_OK;
+}
+
+static int make_batch_taps(SetupTaps &st, Taps &av, int ndx, const float *wx, const int *oxx,
+                           const int *oyx, int ndy, const float *wy, const int *oxy,
+                           const int *oyy, int ndt, const float *wt, const int *oxt,
+                           const int *oyt, int nda, const float *wa, const int *oxa,
+                           const int *oya) {
+    int rc = make_taps(st.dx, ndx, wx, oxx, oyx);
+    if (!rc) rc = make_taps(st.dy, ndy, wy, oxy, oyy);
+    if (!rc) rc = make_taps(st.dt, ndt, wt, oxt, oyt);
+    if (!rc) rc = make_taps(av, nda, wa, oxa, oya);
+    return rc;
+}
+
+extern "C" int hmx_hs_batched(const hmx_pair *pairs, int npairs, int ndx, const float *wx,
+                              const int *oxx, const int *oyx, int ndy, const float *wy,
+                              const int *oxy, const int *oyy, int ndt, const float *wt,
+                              const int *oxt, const int *oyt, int nda, const float *wa,
+                              const int *oxa, const int *oya, int iters, int variant, int k,
+                              int flags, long long spin_limit, void *stream) {
+    SetupTaps st;
+    Taps av;
+    int rc = make_batch_taps(st, av, ndx, wx, oxx, oyx, ndy, wy, oxy, oyy, ndt, wt, oxt, oyt,
+                             nda, wa, oxa, oya);
+    if (rc) return rc;
+    if (spin_limit <= 0) spin_limit = 100000000LL;
+    return batched_impl(pairs, npairs, st, av, iters, variant, k, flags, spin_limit,
+                        as_stream(stream));
+}
+
+extern "C" int hmx_hs_batched_tune(const hmx_pair *pairs, int npairs, int ndx, const float *wx,
+                                   const int *oxx, const int *oyx, int ndy, const float *wy,
+                                   const int *oxy, const int *oyy, int ndt, const float *wt,
+                                   const int *oxt, const int *oyt, int nda, const float *wa,
+                                   const int *oxa, const int *oya, int iters, void *stream,
+                                   int *variant_out, int *k_out) {
+    SetupTaps st;
+    Taps av;
+    int rc = make_batch_taps(st, av, ndx, wx, oxx, oyx, ndy, wy, oxy, oyy, ndt, wt, oxt, oyt,
+                             nda, wa, oxa, oya);
+    if (rc) return rc;
+    hipStream_t s = as_stream(stream);
+    hipEvent_t e0, e1;
+    HMX_HIP_TRY(hipEventCreate(&e0));
+    HMX_HIP_TRY(hipEventCreate(&e1));
+    float best = 1e30f;
+    int bv = 1, bk = 8;
+    for (int vi = 0; vi < kNumBVariants && !rc; ++vi) {
+        for (int k = 2; k <= kBVariants[vi].kmax && !rc; k *= 2) {
+            rc = batched_impl(pairs, npairs, st, av, iters, vi, k, HMX_BATCH_SYNC, 100000000LL, s);
+            if (rc) break;
+            HMX_HIP_TRY(hipEventRecord(e0, s));
+            rc = batched_impl(pairs, npairs, st, av, iters, vi, k, 0, 100000000LL, s);
+            if (rc) break;
+            HMX_HIP_TRY(hipEventRecord(e1, s));
+            HMX_HIP_TRY(hipEventSynchronize(e1));
+            float ms = 0;
+            HMX_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) {
+                best = ms;
+                bv = vi;
+                bk = k;
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    std::vector<int> sig;
+    for (int i = 0; i < npairs; ++i) {
+        sig.push_back(pairs[i].H);
+        sig.push_back(pairs[i].W);
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_bmu);
+        g_btuned.push_back({sig, {bv, bk}});
+    }
+    if (variant_out) *variant_out = bv;
+    if (k_out) *k_out = bk;
+    return HMX_OK;
+}
+
+extern "C" int hmx_hs_batched_release(void) {
+    std::lock_guard<std::mutex> lk(g_bmu);
+    for (auto &p : g_plans) free_plan(p);
+    g_plans.clear();
+    return HMX_OK;
+}

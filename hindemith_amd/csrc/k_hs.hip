@@ -88,11 +88,15 @@ static int tap_class(int nd) {
 static const HsVariant kVariants[] = {
     HS_V(256, 64, 16, 1),     // 0: one iteration per launch (plain sweep)
     HS_V(256, 64, 32, 4),     // 1
-    HS_V(512, 128, 32, 4),    // 2
-    HS_V(512, 64, 64, 8),     // 3
-    HS_V(1024, 128, 64, 8),   // 4: 92 KB LDS
-    HS_V(1024, 256, 32, 4),   // 5: 84 KB LDS
-    HS_V(256, 64, 32, 2),     // 6
+    HS_V(1024, 64, 32, 4),    // 2: 3 px per thread, 8 waves/SIMD
+    HS_V(1024, 64, 32, 8),    // 3
+    HS_V(1024, 128, 32, 8),   // 4
+    HS_V(1024, 128, 64, 8),   // 5: 92 KB LDS
+    HS_V(1024, 64, 64, 4),    // 6
+    HS_V(512, 64, 32, 4),     // 7
+    HS_V(1024, 128, 32, 4),   // 8
+    HS_V(1024, 64, 64, 8),    // 9
+    HS_V(1024, 128, 64, 4),   // 10
 };
 #undef HS_V
 static constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
@@ -101,17 +105,10 @@ static std::mutex g_tune_mu;
 static std::map<std::pair<int, int>, std::pair<int, int>> g_tuned;  // (H,W) -> (variant,k)
 
 static void default_choice(int H, int W, int &variant, int &k) {
-    const int64_t n = (int64_t)H * W;
-    if (n >= (int64_t)1024 * 1024) {
-        variant = 4;
-        k = 8;
-    } else if (n >= 256 * 256) {
-        variant = 3;
-        k = 8;
-    } else {
-        variant = 1;
-        k = 4;
-    }
+    (void)H;
+    (void)W;
+    variant = 3;
+    k = 8;
 }
 
 int launch_hs_tb(int variant, const float *Ix, const float *Iy, const float *It,

@@ -46,17 +46,15 @@ Reference defects fixed (each raised a KeyError, ran pdb or mis-ran before):
 from __future__ import annotations
 
 import ast
-import ctypes
-import inspect
 import itertools
 import os
 import textwrap
 import threading
-import warnings
 
 import numpy as np
 
-from ._lib import HindemithError, check, lib
+from . import backend
+from ._lib import HindemithError
 
 __all__ = ["mycompile", "run_backend", "Record", "device_from_env", "set_plan_only"]
 
@@ -496,7 +494,7 @@ def _ensure_init():
     with _init_lock:
         dev = -1 if _plan_only[0] else device_from_env()
         if _initialised[0] != dev:
-            check(lib().hmx_init(dev), "hmx_init")
+            backend.init(dev)
             _initialised[0] = dev
 
 
@@ -528,12 +526,7 @@ class Record:
         self.consts_xml = consts_xml
 
     def plan(self):
-        buf = ctypes.create_string_buffer(1 << 16)
-        check(lib().hmx_plan_text(self.id, buf, len(buf)), "plan_text")
-        return buf.value.decode()
-
-    def release(self):
-        check(lib().hmx_release(self.id), "release")
+        return backend.plan(self.id)
 
 
 def mycompile(source, kwargs, defines, fn_key=None):
@@ -555,85 +548,13 @@ def mycompile(source, kwargs, defines, fn_key=None):
     st_xml = symbol_table_xml(symtab)
     c_xml = defines_xml(defines)
     rid = next(_rec_ids)
-    check(lib().hmx_compile(rid, st_xml.encode(), model_xml.encode(), c_xml.encode()),
-          "compile")
+    backend.xmlcompile(rid, st_xml, model_xml, c_xml)
     rec = Record(rid, symtab, model_xml, st_xml, c_xml)
     _records[key] = rec
     return rec
 
 
-def _bind(rec, name, comp, arr, keep):
-    arr = np.ascontiguousarray(arr) if not arr.flags["F_CONTIGUOUS"] else arr
-    keep.append(arr)
-    check(lib().hmx_bind(rec.id, name.encode(), comp, arr.ctypes.data_as(ctypes.c_void_p),
-                         arr.nbytes), f"bind {name}")
-
-
 def run_backend(rec, kwargs):
-    """PyAST.run_backend / backend.run: bind kwargs, run, build the return value."""
-    keep = []
-    for name, sym in rec.symtab.items():
-        if sym.properties.get("source") != "Header":
-            continue
-        obj = kwargs.get(name)
-        if obj is None:
-            raise HindemithError(f"kwarg {name} missing (backend.cpp:417-420)")
-        tag = sym.tag
-        if tag in ("Array", "Array2D"):
-            if obj.dtype != np.float32:
-                raise HindemithError(f"kwarg {name}: arrays must be float32 on this backend, "
-                                     f"got {obj.dtype}")
-            if not (obj.flags["C_CONTIGUOUS"] or obj.flags["F_CONTIGUOUS"]):
-                raise HindemithError(f"kwarg {name}: array must be contiguous")
-            _bind(rec, name, 0, obj, keep)
-        elif tag == "Scalar":
-            dt = sym.properties["dtype"]
-            _bind(rec, name, 0, np.array([obj], dtype=dt), keep)
-        elif tag in ("Stencil", "SpMat2D"):
-            data = np.ascontiguousarray(obj.data, dtype=np.float32)
-            if tag == "SpMat2D" and obj.data.dtype != np.float32:
-                raise HindemithError(f"kwarg {name}: SpMat2D planes must be float32")
-            _bind(rec, name, 0, data, keep)
-            _bind(rec, name, 1, np.ascontiguousarray(obj.offx, dtype=np.int32), keep)
-            _bind(rec, name, 2, np.ascontiguousarray(obj.offy, dtype=np.int32), keep)
-        elif tag == "CSRMat":
-            if obj.dtype != np.float32:
-                raise HindemithError(f"kwarg {name}: csr_matrix data must be float32")
-            idx = np.asarray(obj.indices)
-            ptr = np.asarray(obj.indptr)
-            if idx.size and (idx.max() > np.iinfo(np.int32).max or ptr[-1] > np.iinfo(np.int32).max):
-                raise HindemithError(f"kwarg {name}: indices exceed int32 (CSRMat.cpp:26-27)")
-            _bind(rec, name, 0, np.ascontiguousarray(obj.data), keep)
-            _bind(rec, name, 1, np.ascontiguousarray(idx, dtype=np.int32), keep)
-            _bind(rec, name, 2, np.ascontiguousarray(ptr, dtype=np.int32), keep)
-    check(lib().hmx_run(rec.id), "run")
-    n = ctypes.c_int(0)
-    check(lib().hmx_result_count(rec.id, ctypes.byref(n)), "result_count")
-    outs = []
-    for i in range(n.value):
-        kind, dt, nd = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-        shape = (ctypes.c_int64 * 3)()
-        nm = ctypes.c_char_p()
-        check(lib().hmx_result_info(rec.id, i, ctypes.byref(kind), ctypes.byref(dt),
-                                    ctypes.byref(nd), shape, ctypes.byref(nm)), "result_info")
-        name = nm.value.decode()
-        dtype = {0: np.float32, 1: np.float64, 2: np.int64, 3: np.int32}[dt.value]
-        if kind.value == 3:  # scalar -> Python number (backend.cpp:277-306)
-            val = np.zeros(1, dtype=dtype)
-            check(lib().hmx_result_read(rec.id, i, val.ctypes.data_as(ctypes.c_void_p),
-                                        val.nbytes), "result_read")
-            outs.append(int(val[0]) if dtype == np.int64 else float(val[0]))
-            continue
-        shp = (int(shape[0]), int(shape[1])) if nd.value == 2 else (int(shape[0]),)
-        order = "F" if shape[2] else "C"
-        src = kwargs.get(name) if rec.symtab[name].properties.get("source") == "Header" else None
-        if isinstance(src, np.ndarray):
-            out = src  # returned kwarg arrays are updated in place
-        else:
-            out = np.empty(shp, dtype=dtype, order=order)
-        check(lib().hmx_result_read(rec.id, i, out.ctypes.data_as(ctypes.c_void_p), out.nbytes),
-              "result_read")
-        outs.append(out)
-    if not outs:
-        return None
-    return outs[0] if len(outs) == 1 else tuple(outs)
+    """PyAST.run_backend (PyAST.py:198-200): backend.run, keep the return value."""
+    retval, _state = backend.run(rec.id, kwargs)
+    return retval

@@ -22,7 +22,8 @@ import numpy as np
 
 __all__ = [
     "gaussian_blobs", "warp_bilinear", "constant_flow", "affine_flow",
-    "hs_stencil_taps", "laplacian5_taps", "hs_pair", "batched_pairs",
+    "hs_stencil_taps", "laplacian5_taps", "hs_pair", "batched_pairs", "laplacian5_dia_planes",
+    "dia2d_to_csr", "powerlaw_csr",
 ]
 
 
@@ -146,3 +147,46 @@ def batched_pairs(n_pairs=64, seed=0, side=(64, 1024), alpha=(0.5, 2.0)):
         pairs.append(dict(I1=f1.astype(np.float32), I2=f2.astype(np.float32),
                           fx=np.float32(tx), fy=np.float32(ty), alpha=np.float32(a)))
     return pairs
+
+
+def laplacian5_dia_planes(H, W):
+    """The 5-point Laplacian as SpMat2D planes (constant per diagonal)."""
+    w, ox, oy = laplacian5_taps()
+    return np.stack([np.full((H, W), float(v), np.float32) for v in w]), ox, oy
+
+
+def dia2d_to_csr(planes, offx, offy):
+    """CSR (indptr, indices, data; int32/float32) of a 2D DIA matrix with the
+    coefficient read at the neighbour index and out-of-range terms dropped.
+    Entries of a row are ordered by diagonal, as the DIA kernel visits them."""
+    nd, H, W = planes.shape
+    n = H * W
+    rr, cc = np.meshgrid(np.arange(H), np.arange(W), indexing="ij")
+    cols = np.full((n, nd), -1, np.int64)
+    vals = np.zeros((n, nd), np.float32)
+    for d in range(nd):
+        r2, c2 = rr + int(offy[d]), cc + int(offx[d])
+        m = ((r2 >= 0) & (r2 < H) & (c2 >= 0) & (c2 < W)).reshape(-1)
+        q = (r2 * W + c2).reshape(-1)
+        cols[m, d] = q[m]
+        vals[m, d] = planes[d].reshape(-1)[q[m]]
+    keep = cols >= 0
+    indptr = np.zeros(n + 1, np.int64)
+    indptr[1:] = np.cumsum(keep.sum(1))
+    return (indptr.astype(np.int32), cols[keep].astype(np.int32), vals[keep].astype(np.float32))
+
+
+def powerlaw_csr(n_rows, rng, s=1.5, lo=1, hi=512, n_cols=None):
+    """Random CSR: row lengths Zipf(s) truncated to [lo, hi] (sampled from the
+    normalised truncated distribution), columns U[0, n_cols), values N(0, 1)."""
+    n_cols = n_rows if n_cols is None else n_cols
+    k = np.arange(lo, hi + 1, dtype=np.float64)
+    p = k ** -s
+    p /= p.sum()
+    lens = rng.choice(k.astype(np.int64), size=n_rows, p=p)
+    indptr = np.zeros(n_rows + 1, np.int64)
+    indptr[1:] = np.cumsum(lens)
+    nnz = int(indptr[-1])
+    indices = rng.integers(0, n_cols, nnz, dtype=np.int32)
+    data = rng.standard_normal(nnz, dtype=np.float32)
+    return indptr.astype(np.int32), indices, data

@@ -245,6 +245,54 @@ int hmx_hs_solve_f64(const double *I1, const double *I2, int B, int H, int W,
                      int nda, const float *wa, const int *oxa, const int *oya,
                      int iters, double *u, double *v, double *workspace, void *stream);
 
+/* ---------------------------------------------------------------------- */
+/* a14. Batched persistent solve of many pairs of uneven sizes (no          */
+/* counterpart in the reference).                                           */
+
+/* One pair: device frames I1, I2 (H x W), initial/final flow u, v, and a
+ * work area of 5*H*W floats (Ix, Iy, It and the ping-pong planes). */
+struct hmx_pair {
+    const float *I1;
+    const float *I2;
+    float *u;
+    float *v;
+    float *work;
+    int H;
+    int W;
+    float alpha; /* den = alpha^2 + Ix^2 + Iy^2 */
+};
+
+#define HMX_BATCH_GRAPH 1 /* capture the launch in a hipGraph, replay on later calls */
+#define HMX_BATCH_SYNC 2  /* synchronise and report a barrier time-out */
+
+/* Setup + iters Jacobi iterations of every pair in ONE persistent launch: a
+ * grid no larger than the GPU holds at once walks the tiles of all pairs,
+ * with a grid-wide barrier between phases of k iterations.  variant = -1
+ * uses the autotuned (or default) tile shape and k.  Barrier waits give up
+ * after spin_limit polls (<= 0: default) and then every work-group exits;
+ * with HMX_BATCH_SYNC the call returns HMX_ERR_TIMEOUT in that case.
+ * Launch state (descriptors, barrier words, graph) is cached per signature;
+ * calls that overlap on one stream must not share pairs. */
+int hmx_hs_batched(const struct hmx_pair *pairs, int npairs,
+                   int ndx, const float *wx, const int *oxx, const int *oyx,
+                   int ndy, const float *wy, const int *oxy, const int *oyy,
+                   int ndt, const float *wt, const int *oxt, const int *oyt,
+                   int nda, const float *wa, const int *oxa, const int *oya,
+                   int iters, int variant, int k, int flags, long long spin_limit,
+                   void *stream);
+/* Time every (variant, k) on these pairs (their u, v are overwritten) and
+ * cache the best for this list of sizes. */
+int hmx_hs_batched_tune(const struct hmx_pair *pairs, int npairs,
+                        int ndx, const float *wx, const int *oxx, const int *oyx,
+                        int ndy, const float *wy, const int *oxy, const int *oyy,
+                        int ndt, const float *wt, const int *oxt, const int *oyt,
+                        int nda, const float *wa, const int *oxa, const int *oya,
+                        int iters, void *stream, int *variant, int *k);
+int hmx_hs_batched_variant_count(void);
+int hmx_hs_batched_variant_info(int variant, int *tile_w, int *tile_h, int *threads, int *kmax);
+/* Free every cached batched launch (descriptors, graphs). */
+int hmx_hs_batched_release(void);
+
 #ifdef __cplusplus
 }
 #endif

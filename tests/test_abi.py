@@ -13,7 +13,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared_functions():
     hdr = open(os.path.join(ROOT, "include", "hmx.h")).read()
     hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
-    return sorted(set(re.findall(r"\b(?:int|const char \*|void)\s*\*?\s*(hmx_\w+)\s*\(", hdr)))
+    return sorted(set(re.findall(r"\b(?:int|int64_t|const char \*|void)\s*\*?\s*(hmx_\w+)\s*\(",
+                                 hdr)))
 
 
 def test_header_declares_entry_points():

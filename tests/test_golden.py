@@ -1,0 +1,53 @@
+"""The oracle against the committed golden fixtures (tests/golden/golden_hs.npz).
+
+The fixtures come from numpy/scipy code in tests/golden/make_golden.py that
+shares nothing with the C oracle.  fp64 oracle paths must match to rounding;
+fp32 paths within a few fp32 ulps of the operand magnitudes.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from hindemith_amd import synthetic as S
+
+G = np.load(os.path.join(os.path.dirname(__file__), "golden", "golden_hs.npz"))
+TAPS = S.hs_stencil_taps()
+
+
+@pytest.mark.parametrize("name", ["Avg", "Dx", "Dy", "Dt", "Lap5"])
+def test_stencil_fixture(name):
+    st = S.laplacian5_taps() if name == "Lap5" else TAPS[name]
+    x = G["stencil_x"]
+    ref = G[f"stencil_{name}_y"]
+    assert np.allclose(oracle.stencil2d(x, st, np.float64), ref, rtol=0, atol=1e-14)
+    assert np.allclose(oracle.stencil2d(x, st), ref, rtol=0, atol=2e-6)
+
+
+def test_dia2d_fixture():
+    y = oracle.dia2d(G["dia_A"], G["dia_offx"], G["dia_offy"], G["dia_x"], np.float64)
+    assert np.allclose(y, G["dia_y"], rtol=0, atol=1e-13)
+
+
+def test_csr_fixture():
+    y = oracle.csr_spmv(G["csr_indptr"], G["csr_indices"], G["csr_data"], G["csr_x"],
+                        np.float64)
+    assert np.allclose(y, G["csr_y"], rtol=0, atol=1e-13)
+    assert y[5] == 0.0
+
+
+def test_sum_fixture():
+    assert oracle.sum2d(G["sum_x"]) == pytest.approx(float(G["sum_y"]), rel=1e-15)
+
+
+def test_hs_fixture():
+    Ix, Iy, It = oracle.hs_setup(G["hs_I1"], G["hs_I2"], TAPS["Dx"], TAPS["Dy"], TAPS["Dt"],
+                                 np.float64)
+    for a, b in ((Ix, "hs_Ix"), (Iy, "hs_Iy"), (It, "hs_It")):
+        assert np.allclose(a, G[b], atol=1e-14)
+    u, v = oracle.hs_jacobi(Ix, Iy, It, float(G["hs_alpha2"]), TAPS["Avg"], int(G["hs_iters"]),
+                            dtype=np.float64)
+    assert np.allclose(u, G["hs_u"], atol=1e-12) and np.allclose(v, G["hs_v"], atol=1e-12)
+    u32, v32 = oracle.hs_solve(G["hs_I1"], G["hs_I2"], 1.0, TAPS, int(G["hs_iters"]))
+    assert np.max(np.abs(u32 - G["hs_u"])) < 1e-5 and np.max(np.abs(v32 - G["hs_v"])) < 1e-5

@@ -1,0 +1,100 @@
+"""Batched persistent Horn-Schunck (hmx_hs_batched) on the GPU.
+
+Per-pair results must equal the single-pair path and the fp32 oracle bit for
+bit (same tile body, same per-op rounding); graph replays, interleaved with
+other GPU work, must all agree; a host-forced tiny barrier spin limit must
+end in HMX_ERR_TIMEOUT with the grid drained (later calls still work).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from hindemith_amd import synthetic as S
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+TAPS = S.hs_stencil_taps()
+
+
+def make(n, seed, side=(20, 300), iters=23):
+    from hindemith_amd.batched import BatchedHS
+    pairs = S.batched_pairs(n, seed=seed, side=side)
+    B = BatchedHS([torch.from_numpy(p["I1"]).cuda() for p in pairs],
+                  [torch.from_numpy(p["I2"]).cuda() for p in pairs],
+                  [p["alpha"] for p in pairs], iters=iters)
+    return pairs, B
+
+
+def check_oracle(pairs, out, iters):
+    for p, (u, v) in zip(pairs, out):
+        a2 = np.float32(p["alpha"]) * np.float32(p["alpha"])
+        ru, rv = oracle.hs_solve(p["I1"], p["I2"], a2, TAPS, iters)
+        assert np.array_equal(u.cpu().numpy(), ru) and np.array_equal(v.cpu().numpy(), rv)
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batched_matches_oracle(graph):
+    pairs, B = make(9, seed=1)
+    out = B.run(graph=graph, sync=True)
+    torch.cuda.synchronize()
+    check_oracle(pairs, out, 23)
+
+
+def test_all_variants_and_ks():
+    from hindemith_amd._lib import lib
+    pairs, B = make(5, seed=2, iters=17)
+    L = lib()
+    for vi in range(L.hmx_hs_batched_variant_count()):
+        kmax = ctypes_info(vi)
+        for k in sorted({1, 2, kmax}):
+            out = B.run(graph=False, sync=True, variant=vi, k=k)
+            torch.cuda.synchronize()
+            check_oracle(pairs, out, 17)
+
+
+def ctypes_info(vi):
+    import ctypes
+    from hindemith_amd._lib import lib
+    km = ctypes.c_int()
+    lib().hmx_hs_batched_variant_info(vi, None, None, None, ctypes.byref(km))
+    return km.value
+
+
+def test_graph_replays_consistent_with_interleaved_work():
+    pairs, B = make(7, seed=3, iters=30)
+    ref = [(u.clone(), v.clone()) for u, v in B.run(graph=False, sync=True)]
+    x = torch.randn(1 << 20, device="cuda")
+    for rep in range(4):
+        out = B.run(graph=True)
+        y = (x * 2.0).sum()  # unrelated eager work between replays
+        torch.cuda.synchronize()
+        assert float(y) == float(y)
+        for (u, v), (ru, rv) in zip(out, ref):
+            assert torch.equal(u, ru) and torch.equal(v, rv), rep
+
+
+def test_tune_then_run_matches():
+    pairs, B = make(6, seed=4, iters=20)
+    vi, k = B.tune()
+    out = B.run(graph=True, sync=True)
+    torch.cuda.synchronize()
+    check_oracle(pairs, out, 20)
+
+
+def test_forced_barrier_timeout_drains_and_recovers():
+    from hindemith_amd._lib import HindemithError
+    pairs, B = make(12, seed=5, side=(200, 400), iters=16)
+    with pytest.raises(HindemithError, match="gave up"):
+        B.run(graph=False, sync=True, spin_limit=1)
+    out = B.run(graph=False, sync=True)
+    torch.cuda.synchronize()
+    check_oracle(pairs, out, 16)
+
+
+def test_batched_equals_per_pair_plain_solves():
+    pairs, B = make(8, seed=6, iters=25)
+    batched = [(u.clone(), v.clone()) for u, v in B.run(graph=True, sync=True)]
+    plain = B.solve_each()
+    torch.cuda.synchronize()
+    for (bu, bv), (pu, pv) in zip(batched, plain):
+        assert torch.equal(bu, pu) and torch.equal(bv, pv)

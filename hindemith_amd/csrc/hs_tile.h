@@ -28,6 +28,7 @@ struct HsTapTable {
     float w[HS_MAX_TAPS];
     int dl[HS_MAX_TAPS];  // LDS delta oy*PITCH + ox
     int ox[HS_MAX_TAPS], oy[HS_MAX_TAPS];
+    int pos[HS_MAX_TAPS];  // 3x3 window position (oy+1)*3 + (ox+1)
     int nd;
     __device__ __forceinline__ explicit HsTapTable(const Taps &t) {
         nd = t.nd;
@@ -38,9 +39,25 @@ struct HsTapTable {
             ox[d] = on ? t.ox[d] : 0;
             oy[d] = on ? t.oy[d] : 0;
             dl[d] = oy[d] * PITCH + ox[d];
+            pos[d] = (oy[d] + 1) * 3 + (ox[d] + 1);
         }
     }
 };
+
+// The per-pixel update of the canonical program, one rounding per op:
+//   t = ((Ix*ubar + Iy*vbar) + It) / den;  u' = ubar - Ix*t;  v' = vbar - Iy*t
+__device__ __forceinline__ void hs_update(float ix, float iy, float it, float den, float ub,
+                                          float vb, float &un, float &vn) {
+    const float m1 = ix * ub;
+    const float m2 = iy * vb;
+    const float s1 = m1 + m2;
+    const float s2 = s1 + it;
+    const float tt = s2 / den;
+    const float p1 = ix * tt;
+    const float p2 = iy * tt;
+    un = ub - p1;
+    vn = vb - p2;
+}
 
 // Advance output tile t of image a by k (<= KMAX) Jacobi steps.  uv is the
 // work-group's LDS window of (TW + 2*KMAX) x (TH + 2*KMAX) float2, stored with
@@ -131,15 +148,7 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
                         }
                     }
                 }
-                const float m1 = cx[j] * ub;
-                const float m2 = cy[j] * vb;
-                const float s1 = m1 + m2;
-                const float s2 = s1 + ct[j];
-                const float tt = s2 / cd[j];
-                const float p1 = cx[j] * tt;
-                const float p2 = cy[j] * tt;
-                nu[j] = ub - p1;
-                nv[j] = vb - p2;
+                hs_update(cx[j], cy[j], ct[j], cd[j], ub, vb, nu[j], nv[j]);
             }
         }
         __syncthreads();
@@ -162,6 +171,151 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
         }
     }
     __syncthreads();  // the window is reused by the work-group's next tile
+}
+
+
+// Column-strip tile body.  Thread q of the work-group owns S vertically
+// adjacent pixels of one column of the region (strips are laid out column-
+// fastest so a wavefront reads consecutive LDS words).  Per step it reads the
+// 3 x (S+2) float2 window around its strip once -- 3(S+2)/S LDS reads per
+// pixel instead of one per tap -- and applies the taps from registers.  Tap
+// offsets are runtime values; a uniform switch per tap selects the window
+// position, so register indices stay compile-time and the fma order is still
+// the user's tap order.  Window coordinates are clamped to the region, which
+// equals the reference's image clamp on sides where the region meets the image
+// border and is never reached by an updated pixel on the other sides.
+template <int NT, int TW, int TH, int KMAX, int S, int NTAP>
+__device__ __forceinline__ void hs_tile_strip(const HsImage &a, int t, int k,
+                                              const HsTapTable<TW + 2 * KMAX> &tp,
+                                              float2 *uv) {
+    constexpr int PITCH = TW + 2 * KMAX, RHM = TH + 2 * KMAX;
+    constexpr int SR = (RHM + S - 1) / S;
+    constexpr int NS = PITCH * SR;
+    constexpr int M = (NS + NT - 1) / NT;
+    const int tid = threadIdx.x;
+    const int tyi = t / a.tiles_x, txi = t - tyi * a.tiles_x;
+    const int x0 = txi * TW, y0 = tyi * TH;
+    const int x1 = min(x0 + TW, a.W), y1 = min(y0 + TH, a.H);
+    const int gx0 = max(x0 - k, 0), gy0 = max(y0 - k, 0);
+    const int gx1 = min(x1 + k, a.W), gy1 = min(y1 + k, a.H);
+    const int RW = gx1 - gx0, RH = gy1 - gy0;
+
+    float cx[M][S], cy[M][S], ct[M][S], cd[M][S];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const int q = tid + m * NT;
+        const int col = q % PITCH, srow = q / PITCH;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int ry = srow * S + s;
+            cx[m][s] = cy[m][s] = ct[m][s] = 0.0f;
+            cd[m][s] = 1.0f;
+            if (q < NS && col < RW && ry < RH) {
+                const int64_t g = (int64_t)(gy0 + ry) * a.W + gx0 + col;
+                const float ix = a.Ix[g], iy = a.Iy[g];
+                cx[m][s] = ix;
+                cy[m][s] = iy;
+                ct[m][s] = a.It[g];
+                if (a.den) {
+                    cd[m][s] = a.den[g];
+                } else {
+                    const float m1 = ix * ix;
+                    const float s1 = a.alpha2 + m1;
+                    const float m2 = iy * iy;
+                    cd[m][s] = s1 + m2;
+                }
+                uv[ry * PITCH + col] = make_float2(a.u_in[g], a.v_in[g]);
+            }
+        }
+    }
+    __syncthreads();
+
+    for (int it = 1; it <= k; ++it) {
+        const int e = k - it;
+        const int rx0 = max(x0 - e, 0) - gx0, ry0 = max(y0 - e, 0) - gy0;
+        const int rx1 = min(x1 + e, a.W) - gx0, ry1 = min(y1 + e, a.H) - gy0;
+        float nu[M][S], nv[M][S];
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            int q = tid + m * NT;
+            asm volatile("" : "+v"(q));  // keep the window addresses inside the step loop
+            const int col = q % PITCH, srow = q / PITCH;
+            const int cm = min(max(col - 1, 0), RW - 1), c0 = min(col, RW - 1);
+            const int cp = min(col + 1, RW - 1);
+            float2 win[S + 2][3];
+#pragma unroll
+            for (int j = 0; j < S + 2; ++j) {
+                const int r = min(max(srow * S - 1 + j, 0), RH - 1) * PITCH;
+                win[j][0] = uv[r + cm];
+                win[j][1] = uv[r + c0];
+                win[j][2] = uv[r + cp];
+            }
+            float ub[S], vb[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) ub[s] = vb[s] = 0.0f;
+#pragma unroll
+            for (int d = 0; d < (NTAP ? NTAP : HS_MAX_TAPS); ++d) {
+                if (NTAP || d < tp.nd) {
+                    const float w = tp.w[d];
+#define HS_STRIP_TAP(P)                                                       \
+    case P:                                                                   \
+        _Pragma("unroll") for (int s = 0; s < S; ++s) {                      \
+            ub[s] = fmaf(w, win[s + (P) / 3][(P) % 3].x, ub[s]);             \
+            vb[s] = fmaf(w, win[s + (P) / 3][(P) % 3].y, vb[s]);             \
+        }                                                                     \
+        break;
+                    switch (tp.pos[d]) {
+                        HS_STRIP_TAP(0)
+                        HS_STRIP_TAP(1)
+                        HS_STRIP_TAP(2)
+                        HS_STRIP_TAP(3)
+                        HS_STRIP_TAP(4)
+                        HS_STRIP_TAP(5)
+                        HS_STRIP_TAP(6)
+                        HS_STRIP_TAP(7)
+                        HS_STRIP_TAP(8)
+                        default: break;
+                    }
+#undef HS_STRIP_TAP
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+                hs_update(cx[m][s], cy[m][s], ct[m][s], cd[m][s], ub[s], vb[s], nu[m][s],
+                          nv[m][s]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const int q = tid + m * NT;
+            const int col = q % PITCH, srow = q / PITCH;
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const int ry = srow * S + s;
+                if (q < NS && col >= rx0 && col < rx1 && ry >= ry0 && ry < ry1)
+                    uv[ry * PITCH + col] = make_float2(nu[m][s], nv[m][s]);
+            }
+        }
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const int q = tid + m * NT;
+        const int col = q % PITCH, srow = q / PITCH;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int ry = srow * S + s;
+            const int gx = gx0 + col, gy = gy0 + ry;
+            if (q < NS && gx >= x0 && gx < x1 && gy >= y0 && gy < y1 && ry < RH) {
+                const int64_t g = (int64_t)gy * a.W + gx;
+                const float2 v = uv[ry * PITCH + col];
+                a.u_out[g] = v.x;
+                a.v_out[g] = v.y;
+            }
+        }
+    }
+    __syncthreads();
 }
 
 }  // namespace hmx

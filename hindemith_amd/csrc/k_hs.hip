@@ -61,6 +61,27 @@ __global__ __launch_bounds__(NT) void hs_tb_kernel(HsArgs a, Taps taps) {
     hs_tile<NT, TW, TH, KMAX, NTAP>(im, xcd_remap(blockIdx.x, a.ntiles), a.k, tp, uv);
 }
 
+template <int NT, int TW, int TH, int KMAX, int S, int NTAP>
+__global__ __launch_bounds__(NT) void hs_strip_kernel(HsArgs a, Taps taps) {
+    __shared__ float2 uv[(TW + 2 * KMAX) * (TH + 2 * KMAX)];
+    const int64_t off = (int64_t)blockIdx.y * a.bstride;
+    HsImage im;
+    im.Ix = a.Ix + off;
+    im.Iy = a.Iy + off;
+    im.It = a.It + off;
+    im.den = a.den ? a.den + off : nullptr;
+    im.u_in = a.u_in + off;
+    im.v_in = a.v_in + off;
+    im.u_out = a.u_out + off;
+    im.v_out = a.v_out + off;
+    im.alpha2 = a.alpha2_b ? a.alpha2_b[blockIdx.y] : a.alpha2;
+    im.H = a.H;
+    im.W = a.W;
+    im.tiles_x = a.tiles_x;
+    const HsTapTable<TW + 2 * KMAX> tp(taps);
+    hs_tile_strip<NT, TW, TH, KMAX, S, NTAP>(im, xcd_remap(blockIdx.x, a.ntiles), a.k, tp, uv);
+}
+
 // ---------------------------------------------------------------------------
 // Variant table.  (threads, tile, max iterations per launch); LDS use is
 // 8*(TW+2K)(TH+2K) bytes.
@@ -85,6 +106,12 @@ static int tap_class(int nd) {
      {hs_tb_kernel<NT, TW, TH, K, 4>, hs_tb_kernel<NT, TW, TH, K, 5>,                  \
       hs_tb_kernel<NT, TW, TH, K, 8>, hs_tb_kernel<NT, TW, TH, K, 9>,                  \
       hs_tb_kernel<NT, TW, TH, K, 0>}}
+// column-strip variants (S pixels per thread per column)
+#define HS_S(NT, TW, TH, K, S)                                                          \
+    {NT, TW, TH, K,                                                                     \
+     {hs_strip_kernel<NT, TW, TH, K, S, 4>, hs_strip_kernel<NT, TW, TH, K, S, 5>,      \
+      hs_strip_kernel<NT, TW, TH, K, S, 8>, hs_strip_kernel<NT, TW, TH, K, S, 9>,      \
+      hs_strip_kernel<NT, TW, TH, K, S, 0>}}
 static const HsVariant kVariants[] = {
     HS_V(256, 64, 16, 1),     // 0: one iteration per launch (plain sweep)
     HS_V(256, 64, 32, 4),     // 1
@@ -97,8 +124,14 @@ static const HsVariant kVariants[] = {
     HS_V(1024, 128, 32, 4),   // 8
     HS_V(1024, 64, 64, 8),    // 9
     HS_V(1024, 128, 64, 4),   // 10
+    HS_S(1024, 64, 48, 4, 4), // 11: strips, region 72x56
+    HS_S(960, 64, 32, 8, 4),  // 12: region 80x48
+    HS_S(576, 64, 24, 4, 4),  // 13: region 72x32
+    HS_S(1024, 56, 56, 4, 4), // 14: region 64x64
+    HS_S(1024, 120, 24, 4, 4),// 15: region 128x32
 };
 #undef HS_V
+#undef HS_S
 static constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
 static std::mutex g_tune_mu;

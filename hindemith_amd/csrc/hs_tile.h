@@ -16,6 +16,10 @@ struct HsImage {
     float *u_out, *v_out;
     float alpha2;
     int H, W, tiles_x;
+    // Optional: after step j (1..k) the output tile's state is also written to
+    // (su, sv) + (j-1)*sstride (used to recompute checkpoint segments).
+    float *su = nullptr, *sv = nullptr;
+    int64_t sstride = 0;
 };
 
 // Largest averaging stencil the fused kernel takes: any radius-1 stencil with
@@ -155,6 +159,17 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
 #pragma unroll
         for (int j = 0; j < P; ++j)
             if (act[j]) uv[(pk[j] >> 16) * PITCH + (pk[j] & 0xffff)] = make_float2(nu[j], nv[j]);
+        if (a.su && it < k) {  // intermediate state of the output tile
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                const int gx = gx0 + (pk[j] & 0xffff), gy = gy0 + (pk[j] >> 16);
+                if (act[j] && gx >= x0 && gx < x1 && gy >= y0 && gy < y1) {
+                    const int64_t g = (int64_t)gy * a.W + gx + (int64_t)(it - 1) * a.sstride;
+                    a.su[g] = nu[j];
+                    a.sv[g] = nv[j];
+                }
+            }
+        }
         __syncthreads();
     }
 

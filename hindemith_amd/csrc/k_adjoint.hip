@@ -24,6 +24,8 @@
 // backwards.
 #include "hmx_common.h"
 
+#include <algorithm>
+#include <cstring>
 #include <utility>
 
 #pragma clang fp contract(off)
@@ -33,7 +35,8 @@ namespace hmx {
 int launch_hs_tb(int variant, const float *Ix, const float *Iy, const float *It,
                  const float *den, float alpha2, int H, int W, const Taps &taps, int k,
                  const float *u_in, const float *v_in, float *u_out, float *v_out,
-                 hipStream_t s, int B, const float *alpha2_b);
+                 hipStream_t s, int B, const float *alpha2_b, float *su, float *sv,
+                 int64_t sstride);
 int hs_jacobi_run(const float *Ix, const float *Iy, const float *It, const float *den,
                   float alpha2, int H, int W, const Taps &taps, int iters, float *u, float *v,
                   float *wu, float *wv, int variant, int k, hipStream_t s, int B,
@@ -73,111 +76,193 @@ __device__ inline float tap_apply_T(const float *__restrict__ g, int H, int W, i
 }
 
 // ---------------------------------------------------------------------------
-// Reverse Jacobi step, fused: one pass reads x_k (halo 1), g (halo 1), Ix, Iy,
-// It and the three accumulators, writes dL/dx_k and the accumulators.
+// Reverse sweep, temporally blocked: one launch walks m (<= MMAX) reverse
+// steps for an output tile.  The gradient (gu, gv) lives in LDS on the tile
+// grown by m (clipped to the image) and shrinks by one pixel per step, exactly
+// like the forward kernel's region; the accumulators gIx, gIy, gIt of the
+// tile's own pixels stay in registers and are added to memory once per
+// launch; each step stages only its state x_i on the tile grown by one.
+// HBM bytes per pixel-step drop from ~60 (one step per launch) to ~24 (m=4).
 
-struct AdjArgs {
+struct AdjTbArgs {
     const float *Ix, *Iy, *It;
-    const float *u, *v;    // x_k
-    const float *gu, *gv;  // dL/dx_{k+1}
-    float *gu_out, *gv_out;
+    const float *gu_in, *gv_in;  // dL/dx_hi
+    float *gu_out, *gv_out;      // dL/dx_{hi-m}
+    const float *xu[8], *xv[8];  // step j (1..m) uses state x_{hi-j}: xu[j-1], xv[j-1]
     float *gIx, *gIy, *gIt;
-    double *ga2;           // [pair][tile] running sums of -s t
+    double *ga2;                 // [pair][tile] running sums of -s t
     const float *alpha2_b;
-    int H, W, tiles_x, ntiles;
+    int H, W, m, tiles_x, ntiles;
     int64_t bstride;
 };
 
-constexpr int ADJ_TW = 64, ADJ_TH = 8, ADJ_NT = 256;
+constexpr int ADJ_TW = 64, ADJ_TH = 32, ADJ_NT = 1024, ADJ_MMAX = 4;
 
-__global__ __launch_bounds__(ADJ_NT) void hs_adj_kernel(AdjArgs a, Taps taps) {
-    constexpr int RW = ADJ_TW + 2, RH = ADJ_TH + 2, NPX = RW * RH;
-    __shared__ float2 xs[NPX];  // (u_k, v_k) at clamped coordinates
-    __shared__ float2 gs[NPX];  // (g_ubar, g_vbar) at real pixels (0 outside)
-    __shared__ double red[ADJ_NT / 64];
+template <int NT, int TW, int TH, int MMAX>
+__global__ __launch_bounds__(NT) void hs_adj_tb_kernel(AdjTbArgs a, Taps taps) {
+    constexpr int PITCH = TW + 2 * MMAX, RHM = TH + 2 * MMAX, NPX = PITCH * RHM;
+    constexpr int P = (NPX + NT - 1) / NT;
+    constexpr int XW = TW + 2, XH = TH + 2;
+    __shared__ float2 G[NPX];    // gradient wrt the state, on the current region
+    __shared__ float2 GB[NPX];   // (g_ubar, g_vbar) on the current region
+    __shared__ float2 XS[XW * XH];  // state x_i on the tile grown by 1 (clamped)
+    __shared__ double red[NT / 64];
 
     const int b = blockIdx.y;
     const int64_t off = (int64_t)b * a.bstride;
-    const float *Ix = a.Ix + off, *Iy = a.Iy + off, *It = a.It + off;
-    const float *U = a.u + off, *V = a.v + off, *GU = a.gu + off, *GV = a.gv + off;
     const float alpha2 = a.alpha2_b[b];
-    const int H = a.H, W = a.W;
+    const int H = a.H, W = a.W, m = a.m;
     const int t = xcd_remap(blockIdx.x, a.ntiles);
     const int tyi = t / a.tiles_x, txi = t - tyi * a.tiles_x;
-    const int x0 = txi * ADJ_TW, y0 = tyi * ADJ_TH;
-    const int gx0 = x0 - 1, gy0 = y0 - 1;
+    const int x0 = txi * TW, y0 = tyi * TH;
+    const int x1 = min(x0 + TW, W), y1 = min(y0 + TH, H);
+    const int gx0 = max(x0 - m, 0), gy0 = max(y0 - m, 0);
+    const int gx1 = min(x1 + m, W), gy1 = min(y1 + m, H);
+    const int RW = gx1 - gx0, RH = gy1 - gy0, npx = RW * RH;
+    const bool clipped = (x0 - m < 0) || (y0 - m < 0) || (x1 + m > W) || (y1 + m > H);
+    const float *Ix = a.Ix + off, *Iy = a.Iy + off, *It = a.It + off;
 
-    for (int i = threadIdx.x; i < NPX; i += ADJ_NT) {
-        const int ry = i / RW, rx = i - ry * RW;
-        const int gx = gx0 + rx, gy = gy0 + ry;
-        const int64_t gc = (int64_t)clampi(gy, 0, H - 1) * W + clampi(gx, 0, W - 1);
-        xs[i] = make_float2(U[gc], V[gc]);
-        float2 g = make_float2(0.0f, 0.0f);
-        if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
-            const float ix = Ix[gc], iy = Iy[gc], gu = GU[gc], gv = GV[gc];
-            const float den = (alpha2 + ix * ix) + iy * iy;
-            const float s = -(ix * gu + iy * gv) / den;
-            g = make_float2(gu + ix * s, gv + iy * s);
-        }
-        gs[i] = g;
-    }
-    __syncthreads();
-
-    double acc_a2 = 0.0;
-    for (int i = threadIdx.x; i < ADJ_TW * ADJ_TH; i += ADJ_NT) {
-        const int ly = i / ADJ_TW, lx = i - ly * ADJ_TW;
-        const int gx = x0 + lx, gy = y0 + ly;
-        if (gx >= W || gy >= H) continue;
-        const int rx = lx + 1, ry = ly + 1, ri = ry * RW + rx;
-        float ub = 0.0f, vb = 0.0f;
-        for (int d = 0; d < taps.nd; ++d) {
-            const float2 q = xs[ri + taps.oy[d] * RW + taps.ox[d]];
-            ub = fmaf(taps.w[d], q.x, ub);
-            vb = fmaf(taps.w[d], q.y, vb);
-        }
-        const int64_t p = (int64_t)gy * W + gx;
-        const float ix = Ix[p], iy = Iy[p], it = It[p], gu = GU[p], gv = GV[p];
-        const float den = (alpha2 + ix * ix) + iy * iy;
-        const float tt = ((ix * ub + iy * vb) + it) / den;
-        const float s = -(ix * gu + iy * gv) / den;
-        const float st2 = 2.0f * s * tt;
-        a.gIx[off + p] += (-gu * tt + s * ub) - st2 * ix;
-        a.gIy[off + p] += (-gv * tt + s * vb) - st2 * iy;
-        a.gIt[off + p] += s;
-        acc_a2 += (double)(-s * tt);
-        // dL/dx_k = A^T g
-        float au = 0.0f, av = 0.0f;
-        if (gx > 0 && gx < W - 1 && gy > 0 && gy < H - 1) {
-            for (int d = 0; d < taps.nd; ++d) {
-                const float2 q = gs[ri - taps.oy[d] * RW - taps.ox[d]];
-                au = fmaf(taps.w[d], q.x, au);
-                av = fmaf(taps.w[d], q.y, av);
+    int pk[P];
+    float cx[P], cy[P], ct[P], cd[P];
+    float ax[P], ay[P], at[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        const int i = threadIdx.x + j * NT;
+        pk[j] = -1;
+        cx[j] = cy[j] = ct[j] = 0.0f;
+        cd[j] = 1.0f;
+        ax[j] = ay[j] = at[j] = 0.0f;
+        if (i < npx) {
+            const int ry = i / RW, rx = i - ry * RW;
+            pk[j] = (ry << 16) | rx;
+            const int64_t g = (int64_t)(gy0 + ry) * W + gx0 + rx;
+            const float ix = Ix[g], iy = Iy[g];
+            cx[j] = ix;
+            cy[j] = iy;
+            ct[j] = It[g];
+            cd[j] = (alpha2 + ix * ix) + iy * iy;
+            G[ry * PITCH + rx] = make_float2(a.gu_in[off + g], a.gv_in[off + g]);
+            // the accumulators continue from their stored value and take one
+            // step's term at a time, so the sums do not depend on how the
+            // reverse sweep is cut into launches
+            const int gx = gx0 + rx, gy = gy0 + ry;
+            if (gx >= x0 && gx < x1 && gy >= y0 && gy < y1) {
+                ax[j] = a.gIx[off + g];
+                ay[j] = a.gIy[off + g];
+                at[j] = a.gIt[off + g];
             }
-        } else {
-            for (int d = 0; d < taps.nd; ++d) {
-                int lox, hix, loy, hiy;
-                tr_range(gx, taps.ox[d], W, lox, hix);
-                tr_range(gy, taps.oy[d], H, loy, hiy);
-                for (int qy = loy; qy <= hiy; ++qy)
-                    for (int qx = lox; qx <= hix; ++qx) {
-                        const float2 q = gs[(qy - gy0) * RW + (qx - gx0)];
+        }
+    }
+    double tile_a2 = 0.0;        // running per-tile sum, held by thread 0
+    if (threadIdx.x == 0) tile_a2 = a.ga2[(int64_t)b * a.ntiles + t];
+
+    for (int step = 1; step <= m; ++step) {
+        // stage x_i (i = hi - step) on the tile grown by one, clamped
+        const float *XU = a.xu[step - 1] + off, *XV = a.xv[step - 1] + off;
+        for (int i = threadIdx.x; i < XW * XH; i += NT) {
+            const int ly = i / XW, lx = i - ly * XW;
+            const int64_t g = (int64_t)clampi(y0 - 1 + ly, 0, H - 1) * W +
+                              clampi(x0 - 1 + lx, 0, W - 1);
+            XS[i] = make_float2(XU[g], XV[g]);
+        }
+        __syncthreads();
+        // pass A on R_{step-1}: s, (g_ubar, g_vbar); accumulate on the tile
+        double a2 = 0.0;
+        {
+            const int e = m - step + 1;
+            const int rx0 = max(x0 - e, 0) - gx0, ry0 = max(y0 - e, 0) - gy0;
+            const int rx1 = min(x1 + e, W) - gx0, ry1 = min(y1 + e, H) - gy0;
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                int pkj = pk[j];
+                asm volatile("" : "+v"(pkj));
+                const int rx = pkj & 0xffff, ry = pkj >> 16;
+                if (pkj < 0 || rx < rx0 || rx >= rx1 || ry < ry0 || ry >= ry1) continue;
+                const float2 g = G[ry * PITCH + rx];
+                const float s = -(cx[j] * g.x + cy[j] * g.y) / cd[j];
+                GB[ry * PITCH + rx] = make_float2(g.x + cx[j] * s, g.y + cy[j] * s);
+                const int gx = gx0 + rx, gy = gy0 + ry;
+                if (gx >= x0 && gx < x1 && gy >= y0 && gy < y1) {
+                    const float2 *xb = XS + (gy - y0 + 1) * XW + (gx - x0 + 1);
+                    float ub = 0.0f, vb = 0.0f;
+                    for (int d = 0; d < taps.nd; ++d) {
+                        const float2 q = xb[taps.oy[d] * XW + taps.ox[d]];
+                        ub = fmaf(taps.w[d], q.x, ub);
+                        vb = fmaf(taps.w[d], q.y, vb);
+                    }
+                    const float tt = ((cx[j] * ub + cy[j] * vb) + ct[j]) / cd[j];
+                    const float st2 = 2.0f * s * tt;
+                    ax[j] += (-g.x * tt + s * ub) - st2 * cx[j];
+                    ay[j] += (-g.y * tt + s * vb) - st2 * cy[j];
+                    at[j] += s;
+                    a2 += (double)(-s * tt);
+                }
+            }
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) a2 += __shfl_xor(a2, o, 64);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a2;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double r = 0.0;
+            for (int i = 0; i < NT / 64; ++i) r += red[i];
+            tile_a2 += r;
+        }
+        // pass B on R_step: g <- A^T g_b (gather over sources that clamp onto p)
+        {
+            const int e = m - step;
+            const int rx0 = max(x0 - e, 0) - gx0, ry0 = max(y0 - e, 0) - gy0;
+            const int rx1 = min(x1 + e, W) - gx0, ry1 = min(y1 + e, H) - gy0;
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                int pkj = pk[j];
+                asm volatile("" : "+v"(pkj));
+                const int rx = pkj & 0xffff, ry = pkj >> 16;
+                if (pkj < 0 || rx < rx0 || rx >= rx1 || ry < ry0 || ry >= ry1) continue;
+                const int gx = gx0 + rx, gy = gy0 + ry;
+                float au = 0.0f, av = 0.0f;
+                if (!clipped || (gx > 0 && gx < W - 1 && gy > 0 && gy < H - 1)) {
+                    const float2 *gb = GB + ry * PITCH + rx;
+                    for (int d = 0; d < taps.nd; ++d) {
+                        const float2 q = gb[-(taps.oy[d] * PITCH + taps.ox[d])];
                         au = fmaf(taps.w[d], q.x, au);
                         av = fmaf(taps.w[d], q.y, av);
                     }
+                } else {
+                    for (int d = 0; d < taps.nd; ++d) {
+                        int lox, hix, loy, hiy;
+                        tr_range(gx, taps.ox[d], W, lox, hix);
+                        tr_range(gy, taps.oy[d], H, loy, hiy);
+                        for (int qy = loy; qy <= hiy; ++qy)
+                            for (int qx = lox; qx <= hix; ++qx) {
+                                const float2 q = GB[(qy - gy0) * PITCH + (qx - gx0)];
+                                au = fmaf(taps.w[d], q.x, au);
+                                av = fmaf(taps.w[d], q.y, av);
+                            }
+                    }
+                }
+                G[ry * PITCH + rx] = make_float2(au, av);
             }
         }
-        a.gu_out[off + p] = au;
-        a.gv_out[off + p] = av;
+        __syncthreads();
     }
+
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) acc_a2 += __shfl_xor(acc_a2, o, 64);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc_a2;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double r = 0.0;
-        for (int i = 0; i < ADJ_NT / 64; ++i) r += red[i];
-        a.ga2[(int64_t)b * a.ntiles + t] += r;
+    for (int j = 0; j < P; ++j) {
+        if (pk[j] < 0) continue;
+        const int rx = pk[j] & 0xffff, ry = pk[j] >> 16;
+        const int gx = gx0 + rx, gy = gy0 + ry;
+        if (gx >= x0 && gx < x1 && gy >= y0 && gy < y1) {
+            const int64_t g = off + (int64_t)gy * W + gx;
+            const float2 q = G[ry * PITCH + rx];
+            a.gu_out[g] = q.x;
+            a.gv_out[g] = q.y;
+            a.gIx[g] = ax[j];
+            a.gIy[g] = ay[j];
+            a.gIt[g] = at[j];
+        }
     }
+    if (threadIdx.x == 0) a.ga2[(int64_t)b * a.ntiles + t] = tile_a2;
 }
 
 // dL/dI1, dL/dI2 from the derivative-cube accumulators.
@@ -254,6 +339,11 @@ static int make_all_taps(AdjTaps &T, int ndx, const float *wx, const int *oxx, c
 static int64_t adj_tiles(int H, int W) {
     return (int64_t)((W + ADJ_TW - 1) / ADJ_TW) * ((H + ADJ_TH - 1) / ADJ_TH);
 }
+
+// Variant of the forward kernel used to recompute a checkpoint segment: it
+// writes every intermediate state of each k-step launch.
+constexpr int ADJ_RECOMPUTE_VARIANT = 3;  // 1024 threads, 64x32 tiles, k <= 8
+constexpr int ADJ_RECOMPUTE_K = 8;
 
 }  // namespace hmx
 
@@ -358,7 +448,8 @@ extern "C" int hmx_hs_adjoint(const float *I1, const float *I2, int B, int H, in
     HMX_HIP_TRY(hipMemcpyAsync(g[0][1], gv_N, n * 4, hipMemcpyDeviceToDevice, s));
     int cur = 0;
 
-    AdjArgs a;
+    AdjTbArgs a;
+    std::memset(&a, 0, sizeof(a));
     a.Ix = Ix;
     a.Iy = Iy;
     a.It = It;
@@ -383,22 +474,34 @@ extern "C" int hmx_hs_adjoint(const float *I1, const float *I2, int B, int H, in
             return j == 0 ? (float *)ckpt + (int64_t)(2 * c + 1) * n
                           : seg + (int64_t)(2 * (j - 1) + 1) * n;
         };
-        for (int j = 1; j < len; ++j) {  // recompute the segment (one step per launch)
-            rc = launch_hs_tb(0, Ix, Iy, It, nullptr, 0.0f, H, W, T.av, 1, state_u(j - 1),
-                              state_v(j - 1), state_u(j), state_v(j), s, B, alpha2);
+        // recompute states 1..len-1 of the segment, k steps per launch, every
+        // intermediate state written out (bit-identical to the forward pass)
+        for (int j0 = 0; j0 < len - 1;) {
+            const int kk = std::min(ADJ_RECOMPUTE_K, len - 1 - j0);
+            rc = launch_hs_tb(ADJ_RECOMPUTE_VARIANT, Ix, Iy, It, nullptr, 0.0f, H, W, T.av, kk,
+                              state_u(j0), state_v(j0), state_u(j0 + kk), state_v(j0 + kk), s, B,
+                              alpha2, kk > 1 ? state_u(j0 + 1) : nullptr,
+                              kk > 1 ? state_v(j0 + 1) : nullptr, 2 * n);
             if (rc) return rc;
+            j0 += kk;
         }
-        for (int j = len - 1; j >= 0; --j) {
-            a.u = state_u(j);
-            a.v = state_v(j);
-            a.gu = g[cur][0];
-            a.gv = g[cur][1];
+        // reverse steps len-1 .. 0, up to ADJ_MMAX per launch
+        for (int hi = len; hi > 0;) {
+            const int mm = std::min(ADJ_MMAX, hi);
+            a.m = mm;
+            for (int st = 1; st <= mm; ++st) {
+                a.xu[st - 1] = state_u(hi - st);
+                a.xv[st - 1] = state_v(hi - st);
+            }
+            a.gu_in = g[cur][0];
+            a.gv_in = g[cur][1];
             a.gu_out = g[cur ^ 1][0];
             a.gv_out = g[cur ^ 1][1];
-            hipLaunchKernelGGL(hs_adj_kernel, dim3((unsigned)ntiles, B), dim3(ADJ_NT), 0, s, a,
-                               T.av);
-            HMX_LAUNCH_CHECK("hs_adj");
+            hipLaunchKernelGGL((hs_adj_tb_kernel<ADJ_NT, ADJ_TW, ADJ_TH, ADJ_MMAX>),
+                               dim3((unsigned)ntiles, B), dim3(ADJ_NT), 0, s, a, T.av);
+            HMX_LAUNCH_CHECK("hs_adj_tb");
             cur ^= 1;
+            hi -= mm;
         }
     }
     if (gu0) HMX_HIP_TRY(hipMemcpyAsync(gu0, g[cur][0], n * 4, hipMemcpyDeviceToDevice, s));

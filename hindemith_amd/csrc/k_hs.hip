@@ -38,6 +38,8 @@ struct HsArgs {
     float alpha2;
     int H, W, k, tiles_x, ntiles;
     int64_t bstride;        // elements between consecutive pairs (blockIdx.y)
+    float *su, *sv;         // optional intermediate-state output (see HsImage)
+    int64_t sstride;
 };
 
 template <int NT, int TW, int TH, int KMAX, int NTAP>
@@ -57,6 +59,9 @@ __global__ __launch_bounds__(NT) void hs_tb_kernel(HsArgs a, Taps taps) {
     im.H = a.H;
     im.W = a.W;
     im.tiles_x = a.tiles_x;
+    im.su = a.su ? a.su + off : nullptr;
+    im.sv = a.sv ? a.sv + off : nullptr;
+    im.sstride = a.sstride;
     const HsTapTable<TW + 2 * KMAX> tp(taps);
     hs_tile<NT, TW, TH, KMAX, NTAP>(im, xcd_remap(blockIdx.x, a.ntiles), a.k, tp, uv);
 }
@@ -147,10 +152,14 @@ static void default_choice(int H, int W, int &variant, int &k) {
 int launch_hs_tb(int variant, const float *Ix, const float *Iy, const float *It,
                  const float *den, float alpha2, int H, int W, const Taps &taps, int k,
                  const float *u_in, const float *v_in, float *u_out, float *v_out,
-                 hipStream_t s, int B = 1, const float *alpha2_b = nullptr) {
+                 hipStream_t s, int B = 1, const float *alpha2_b = nullptr,
+                 float *su = nullptr, float *sv = nullptr, int64_t sstride = 0) {
     const HsVariant &V = kVariants[variant];
     HsArgs a;
     a.alpha2_b = alpha2_b;
+    a.su = su;
+    a.sv = sv;
+    a.sstride = sstride;
     a.bstride = (int64_t)H * W;
     a.Ix = Ix;
     a.Iy = Iy;

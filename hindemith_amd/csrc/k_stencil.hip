@@ -18,7 +18,7 @@ namespace hmx {
 // construction: window cell (rx, ry) holds x[clamp(x0-R+rx), clamp(y0-R+ry)].
 constexpr int ST_TW = 64, ST_TH = 16;
 
-template <int R>
+template <int R, int NTAP>  // NTAP > 0: exactly NTAP taps, unrolled; 0: any count
 __global__ __launch_bounds__(256) void stencil2d_lds_kernel(const float *__restrict__ x,
                                                             float *__restrict__ y, int H, int W,
                                                             Taps taps) {
@@ -31,6 +31,15 @@ __global__ __launch_bounds__(256) void stencil2d_lds_kernel(const float *__restr
         int gx = clampi(x0 - R + rx, 0, W - 1), gy = clampi(y0 - R + ry, 0, H - 1);
         tile[i] = x[(int64_t)gy * W + gx];
     }
+    // taps hoisted into registers once: LDS delta and weight
+    constexpr int NT = NTAP ? NTAP : 1;
+    int dl[NT];
+    float wt[NT];
+#pragma unroll
+    for (int d = 0; d < NT; ++d) {
+        dl[d] = taps.oy[d] * LW + taps.ox[d];
+        wt[d] = taps.w[d];
+    }
     __syncthreads();
     const int gx = x0 + tx;
     if (gx >= W) return;
@@ -39,8 +48,13 @@ __global__ __launch_bounds__(256) void stencil2d_lds_kernel(const float *__restr
         if (gy >= H) break;
         const float *base = tile + (r + R) * LW + tx + R;
         float acc = 0.0f;
-        for (int d = 0; d < taps.nd; ++d)
-            acc = fmaf(taps.w[d], base[taps.oy[d] * LW + taps.ox[d]], acc);
+        if (NTAP) {
+#pragma unroll
+            for (int d = 0; d < NT; ++d) acc = fmaf(wt[d], base[dl[d]], acc);
+        } else {
+            for (int d = 0; d < taps.nd; ++d)
+                acc = fmaf(taps.w[d], base[taps.oy[d] * LW + taps.ox[d]], acc);
+        }
         y[(int64_t)gy * W + gx] = acc;
     }
 }
@@ -66,12 +80,15 @@ int launch_stencil2d(const float *x, float *y, int H, int W, const Taps &t, hipS
     const int R = t.rx > t.ry ? t.rx : t.ry;
     if (R <= 4) {
         dim3 grid((W + ST_TW - 1) / ST_TW, (H + ST_TH - 1) / ST_TH);
-        if (R <= 1)
-            hipLaunchKernelGGL(stencil2d_lds_kernel<1>, grid, dim3(256), 0, s, x, y, H, W, t);
-        else if (R <= 2)
-            hipLaunchKernelGGL(stencil2d_lds_kernel<2>, grid, dim3(256), 0, s, x, y, H, W, t);
-        else
-            hipLaunchKernelGGL(stencil2d_lds_kernel<4>, grid, dim3(256), 0, s, x, y, H, W, t);
+        using KFn = void (*)(const float *, float *, int, int, Taps);
+#define ST_ROW(RR) {stencil2d_lds_kernel<RR, 4>, stencil2d_lds_kernel<RR, 5>,          \
+                    stencil2d_lds_kernel<RR, 8>, stencil2d_lds_kernel<RR, 9>,          \
+                    stencil2d_lds_kernel<RR, 0>}
+        static const KFn table[3][5] = {ST_ROW(1), ST_ROW(2), ST_ROW(4)};
+#undef ST_ROW
+        const int ri = R <= 1 ? 0 : (R <= 2 ? 1 : 2);
+        const int ci = t.nd == 4 ? 0 : t.nd == 5 ? 1 : t.nd == 8 ? 2 : t.nd == 9 ? 3 : 4;
+        hipLaunchKernelGGL(table[ri][ci], grid, dim3(256), 0, s, x, y, H, W, t);
     } else {
         dim3 grid((W + 255) / 256, H < 65535 ? H : 65535);
         hipLaunchKernelGGL(stencil2d_direct_kernel, grid, dim3(256), 0, s, x, y, H, W, t);
@@ -85,6 +102,9 @@ int launch_stencil2d(const float *x, float *y, int H, int W, const Taps &t, hipS
 // (SpmvDIA2D.cpp:41-43; coefficient index SpMat2D.cpp:214).  Each coefficient
 // plane element is read exactly once over the whole launch, so the planes are
 // streamed straight from HBM; the nd shifted reads of x hit in L1/L2.
+// ND > 0: exactly ND diagonals, all loads of a pixel issued before its fma
+// chain; ND == 0: any count (runtime loop).
+template <int ND>
 __global__ __launch_bounds__(256) void dia2d_kernel(const float *__restrict__ A,
                                                     const float *__restrict__ x,
                                                     float *__restrict__ y, int H, int W,
@@ -95,11 +115,27 @@ __global__ __launch_bounds__(256) void dia2d_kernel(const float *__restrict__ A,
     for (int r = blockIdx.y; r < H; r += gridDim.y) {
         const int64_t p = (int64_t)r * W + c;
         float acc = 0.0f;
-        for (int d = 0; d < taps.nd; ++d) {
-            const int xx = c + taps.ox[d], yy = r + taps.oy[d];
-            if (xx >= 0 && xx < W && yy >= 0 && yy < H) {
-                const int64_t q = (int64_t)yy * W + xx;
-                acc = fmaf(A[d * HW + q], x[q], acc);
+        if (ND) {
+            float av[ND ? ND : 1], xv[ND ? ND : 1];
+            bool in[ND ? ND : 1];
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+                const int xx = c + taps.ox[d], yy = r + taps.oy[d];
+                in[d] = xx >= 0 && xx < W && yy >= 0 && yy < H;
+                const int64_t q = in[d] ? (int64_t)yy * W + xx : p;
+                av[d] = A[d * HW + q];
+                xv[d] = x[q];
+            }
+#pragma unroll
+            for (int d = 0; d < ND; ++d)
+                if (in[d]) acc = fmaf(av[d], xv[d], acc);
+        } else {
+            for (int d = 0; d < taps.nd; ++d) {
+                const int xx = c + taps.ox[d], yy = r + taps.oy[d];
+                if (xx >= 0 && xx < W && yy >= 0 && yy < H) {
+                    const int64_t q = (int64_t)yy * W + xx;
+                    acc = fmaf(A[d * HW + q], x[q], acc);
+                }
             }
         }
         y[p] = acc;
@@ -155,7 +191,12 @@ int launch_hs_setup(const float *I1, const float *I2, int H, int W, const Taps &
 int launch_dia2d(const float *A, const float *x, float *y, int H, int W, const Taps &t,
                  hipStream_t s) {
     dim3 grid((W + 255) / 256, H < 65535 ? H : 65535);
-    hipLaunchKernelGGL(dia2d_kernel, grid, dim3(256), 0, s, A, x, y, H, W, t);
+    if (t.nd == 5)
+        hipLaunchKernelGGL(dia2d_kernel<5>, grid, dim3(256), 0, s, A, x, y, H, W, t);
+    else if (t.nd == 9)
+        hipLaunchKernelGGL(dia2d_kernel<9>, grid, dim3(256), 0, s, A, x, y, H, W, t);
+    else
+        hipLaunchKernelGGL(dia2d_kernel<0>, grid, dim3(256), 0, s, A, x, y, H, W, t);
     HMX_LAUNCH_CHECK("dia2d");
     return HMX_OK;
 }

@@ -34,23 +34,36 @@ def K():
     return kernels
 
 
+BOX9 = (np.linspace(0.05, 0.2, 9).astype(np.float32), np.array([-1, 0, 1] * 3, np.int32),
+        np.repeat(np.array([-1, 0, 1], np.int32), 3))
+SEVEN_R2 = (np.array([0.3, -0.2, 0.1, 0.25, 0.05, 0.4, -0.1], np.float32),
+            np.array([0, 2, -2, 1, 0, -1, 0], np.int32), np.array([0, 0, 1, -2, 2, 1, -1], np.int32))
+
+
 @pytest.mark.parametrize("shape", SHAPES)
-@pytest.mark.parametrize("st", ["Avg", "Dx", "Dt", "Lap5", "Wide"])
+@pytest.mark.parametrize("st", ["Avg", "Dx", "Dt", "Lap5", "Wide", "Box9", "SevenR2"])
 def test_stencil2d_bitexact(K, shape, st):
     rng = np.random.default_rng(hash((shape, st)) % 2**32)
     x = rng.uniform(-1, 1, shape).astype(np.float32)
-    taps = {"Lap5": S.laplacian5_taps(), "Wide": WIDE}.get(st) or TAPS[st]
+    taps = {"Lap5": S.laplacian5_taps(), "Wide": WIDE, "Box9": BOX9,
+            "SevenR2": SEVEN_R2}.get(st) or TAPS[st]
     y = host(K.stencil2d(dev(x), taps))
     ref = oracle.stencil2d(x, taps)
     assert np.array_equal(y, ref)
 
 
 @pytest.mark.parametrize("shape", [(1, 1), (37, 41), (256, 256), (99, 517)])
-def test_dia2d_bitexact(K, shape):
+@pytest.mark.parametrize("nd", [3, 5, 9])
+def test_dia2d_bitexact(K, shape, nd):
     rng = np.random.default_rng(1)
     H, W = shape
-    w, ox, oy = S.laplacian5_taps()
-    A = rng.normal(size=(5, H, W)).astype(np.float32)
+    if nd == 5:
+        w, ox, oy = S.laplacian5_taps()
+    elif nd == 9:
+        w, ox, oy = BOX9
+    else:
+        ox, oy = np.array([0, 3, -1], np.int32), np.array([0, 0, 2], np.int32)
+    A = rng.normal(size=(nd, H, W)).astype(np.float32)
     x = rng.uniform(-1, 1, shape).astype(np.float32)
     y = host(K.dia2d_spmv(dev(A), ox, oy, dev(x)))
     assert np.array_equal(y, oracle.dia2d(A, ox, oy, x))

@@ -48,6 +48,11 @@ def parse():
                    help="skip the secondary workloads (4K, bandwidth kernels, batched, adjoint)")
     p.add_argument("--copy-mb", type=int, default=0,
                    help="only run a plain device copy of this many MiB (counter calibration)")
+    p.add_argument("--secondary", choices=["4k", "bandwidth", "batched", "adjoint"],
+                   default=None, help="only run this secondary workload and print its JSON")
+    p.add_argument("--traffic-sweep", action="store_true",
+                   help="only launch every (variant, k) of the fused Jacobi kernel 3 times, in "
+                        "the printed order (for per-configuration rocprofv3 counters)")
     return p.parse_args()
 
 
@@ -62,6 +67,31 @@ def copy_calibration(mb, reps=5):
         y.copy_(x)
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "reps": reps}))
+
+
+def traffic_sweep(H, W):
+    """Three single launches of every (variant, k in {4, kmax}) on the headline
+    input, in the order printed; rocprofv3 --pmc over this run gives each
+    configuration's FETCH_SIZE / WRITE_SIZE per dispatch."""
+    import torch
+    from hindemith_amd import kernels as K
+    from hindemith_amd import synthetic as S
+    taps = S.hs_stencil_taps()
+    I1h, I2h, _, _ = S.hs_pair(H, W, seed=0, flow="affine")
+    I1, I2 = torch.from_numpy(I1h).cuda(), torch.from_numpy(I2h).cuda()
+    ws = torch.empty((5, H, W), device="cuda")
+    u = torch.zeros((H, W), device="cuda")
+    v = torch.zeros_like(u)
+    Ix, Iy, It = K.hs_setup(I1, I2, taps["Dx"], taps["Dy"], taps["Dt"], out=ws[0:3])
+    order = []
+    for vi, info in enumerate(K.hs_variants()):
+        for k in sorted({min(4, info["kmax"]), info["kmax"]}):
+            for _ in range(3):
+                K.hs_jacobi(Ix, Iy, It, 1.0, taps["Avg"], k, u, v, work=ws[3:5], variant=vi,
+                            k=k)
+            torch.cuda.synchronize()
+            order.append({"variant": vi, "k": k, "kernel": info["kernel"], "launches": 3})
+    print(json.dumps({"H": H, "W": W, "order": order}))
 
 
 def cpu_baseline(I1, I2, taps, H, W):
@@ -159,7 +189,10 @@ def extra_bandwidth(K, S):
 
 def extra_batched(S):
     """64 uneven pairs (sides U(64,1024)), 200 iterations: one persistent
-    graph-captured launch vs 64 plain per-pair solves."""
+    graph-captured launch vs 64 plain per-pair solves.  Two plain baselines:
+    the temporally blocked solve of each pair (speedup), and each pair with one
+    fused Jacobi launch per iteration, the reference's launch structure
+    (speedup_vs_launch_per_iteration)."""
     import torch
     from hindemith_amd.batched import BatchedHS
     pairs = S.batched_pairs(64, seed=0)
@@ -170,12 +203,17 @@ def extra_batched(S):
     B.run(graph=True, sync=True)
     ms_b = _time_ms(lambda: B.run(graph=True), 5)
     ms_p = _time_ms(B.solve_each, 3, warm=1)
+    ms_1 = _time_ms(lambda: B.solve_each(variant=0, k=1), 2, warm=1)
     px = sum(p["I1"].size for p in pairs)
     return {"pairs": 64, "megapixels": round(px / 1e6, 2), "iters": 200,
             "batched_pairs_per_s": round(64 / (ms_b * 1e-3), 1),
             "plain_pairs_per_s": round(64 / (ms_p * 1e-3), 1),
             "speedup": round(ms_p / ms_b, 2), "batched_ms": round(ms_b, 3),
-            "plain_ms": round(ms_p, 3), "variant": vi, "k": k}
+            "plain_ms": round(ms_p, 3),
+            "plain_launch_per_iteration_ms": round(ms_1, 3),
+            "speedup_vs_launch_per_iteration": round(ms_1 / ms_b, 2),
+            "batched_mpx_iter_per_s": round(px * 200 / (ms_b * 1e-3) / 1e6, 1),
+            "variant": vi, "k": k}
 
 
 def extra_adjoint(S):
@@ -225,21 +263,25 @@ def extra_adjoint(S):
 
 def load_traffic(variant, k, H, W):
     """HBM bytes per launch from the committed rocprofv3 PMC summary, if it
-    was measured for this exact configuration."""
+    was measured for this exact configuration (variant, k, H, W)."""
     path = os.path.join(HERE, "profiles", "hs_jacobi_1920x1080_pmc.json")
     try:
         d = json.load(open(path))
     except (OSError, ValueError):
         return None
-    if d.get("variant") == variant and d.get("k") == k and d.get("H") == H and d.get("W") == W:
-        return d.get("hbm_bytes_per_launch")
-    return None
+    if d.get("H") != H or d.get("W") != W:
+        return None
+    e = d.get("configs", {}).get(f"v{variant}k{k}")
+    return e.get("hbm_bytes_per_launch") if e else None
 
 
 def main():
     args = parse()
     if args.copy_mb:
         copy_calibration(args.copy_mb)
+        return 0
+    if args.traffic_sweep:
+        traffic_sweep(args.height, args.width)
         return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (the "
@@ -251,6 +293,11 @@ def main():
 
     H, W, iters = args.height, args.width, args.iters
     taps = S.hs_stencil_taps()
+    if args.secondary:
+        fn = {"4k": lambda: extra_4k(K, S, taps), "bandwidth": lambda: extra_bandwidth(K, S),
+              "batched": lambda: extra_batched(S), "adjoint": lambda: extra_adjoint(S)}
+        print(json.dumps({args.secondary: fn[args.secondary]()}))
+        return 0
     I1h, I2h, fx, fy = S.hs_pair(H, W, seed=0, flow="affine")
     dev = torch.device("cuda:0")
     I1 = torch.from_numpy(I1h).to(dev)
@@ -340,7 +387,8 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": load_traffic(variant, k, H, W),
-                "kernel": "hs_tb_kernel", "launches_per_step": launches,
+                "kernel": vinfo["kernel"] if vinfo else "hs_tb_kernel",
+                "launches_per_step": launches,
                 "avg_launch_us": round(jac_ms * 1e3 / launches, 2) if launches else None,
                 "algorithmic_bytes_per_launch": BYTES_PER_PX_ITER * H * W * (k or 1)}
 

@@ -55,12 +55,14 @@ _SIGS = {
                           vp, vp, vp, vp, c_int, c_int, vp],
     "hmx_hs_variant_count": [],
     "hmx_hs_variant_info": [c_int, pint, pint, pint, pint],
+    "hmx_hs_variant_strip": [c_int],
     "hmx_hs_tune": [vp, vp, vp, c_float, c_int, c_int, c_int, vp, vp, vp, vp, vp, vp, vp,
                     vp, pint, pint],
     "hmx_hs_solve": [vp, vp, c_int, c_int, c_float] + [c_int, vp, vp, vp] * 4 +
                     [c_int, vp, vp, vp, vp],
     "hmx_hs_batched_variant_count": [],
     "hmx_hs_batched_variant_info": [c_int, pint, pint, pint, pint],
+    "hmx_hs_batched_resident": [c_int, c_int, pint, pint, pint],
     "hmx_hs_batched_release": [],
     "hmx_hs_ckpt_count": [c_int, c_int],
     "hmx_hs_forward_ckpt": [vp, vp, c_int, c_int, c_int, vp] + [c_int, vp, vp, vp] * 4 +

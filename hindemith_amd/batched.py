@@ -44,6 +44,22 @@ def _sig():
     return L
 
 
+def batched_variants(ntaps=8):
+    """Tile shape, threads, max k and residency of every batched variant."""
+    L = lib()
+    out = []
+    for i in range(L.hmx_hs_batched_variant_count()):
+        tw, th, nt, km, api, own, cus = (ctypes.c_int() for _ in range(7))
+        check(L.hmx_hs_batched_variant_info(i, ctypes.byref(tw), ctypes.byref(th),
+                                            ctypes.byref(nt), ctypes.byref(km)),
+              "batched_variant_info")
+        check(L.hmx_hs_batched_resident(i, ntaps, ctypes.byref(api), ctypes.byref(own),
+                                        ctypes.byref(cus)), "batched_resident")
+        out.append(dict(tile_w=tw.value, tile_h=th.value, threads=nt.value, kmax=km.value,
+                        api_per_cu=api.value, own_per_cu=own.value, cus=cus.value))
+    return out
+
+
 class BatchedHS:
     def __init__(self, I1s, I2s, alphas, iters=200, taps=None):
         if taps is None:
@@ -103,16 +119,26 @@ class BatchedHS:
         self.variant, self.k = vo.value, ko.value
         return self.variant, self.k
 
-    def solve_each(self):
-        """The plain path: one hmx_hs_solve per pair (setup + fused Jacobi)."""
-        from .kernels import hs_solve
+    def solve_each(self, variant=None, k=None):
+        """The plain path: one solve per pair (setup + fused Jacobi).  With
+        variant/k given, the Jacobi launches use that kernel variant (variant 0,
+        k = 1 is one launch per iteration, the reference's launch structure)."""
+        from .kernels import hs_jacobi, hs_setup, hs_solve
+        t = self.taps_dict
         out = []
         for i in range(len(self.I1s)):
             self.u[i].zero_()
             self.v[i].zero_()
             a = self.alphas[i]
             a2 = float(torch.tensor(a, dtype=torch.float32) * torch.tensor(a, dtype=torch.float32))
-            hs_solve(self.I1s[i], self.I2s[i], a2, self.taps_dict, self.iters, self.u[i],
-                     self.v[i], workspace=self.work[i])
+            if variant is None:
+                hs_solve(self.I1s[i], self.I2s[i], a2, t, self.iters, self.u[i], self.v[i],
+                         workspace=self.work[i])
+            else:
+                w = self.work[i]
+                Ix, Iy, It = hs_setup(self.I1s[i], self.I2s[i], t["Dx"], t["Dy"], t["Dt"],
+                                      out=w[0:3])
+                hs_jacobi(Ix, Iy, It, a2, t["Avg"], self.iters, self.u[i], self.v[i],
+                          work=w[3:5], variant=variant, k=k)
             out.append((self.u[i], self.v[i]))
         return out

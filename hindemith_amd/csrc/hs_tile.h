@@ -68,7 +68,10 @@ __device__ __forceinline__ void hs_update(float ix, float iy, float it, float de
 // the fixed row pitch TW + 2*KMAX so every tap is a constant LDS delta.
 // NTAP > 0: the stencil has exactly NTAP taps (straight-line code, batched LDS
 // reads); NTAP == 0: any count up to HS_MAX_TAPS, tested per tap.
-template <int NT, int TW, int TH, int KMAX, int NTAP>
+// SOUT: also write the intermediate states 1..k-1 of the output tile to
+// a.su/a.sv (a compile-time switch: the stores cost registers in every
+// kernel that carries them).
+template <int NT, int TW, int TH, int KMAX, int NTAP, bool SOUT = false>
 __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
                                         const HsTapTable<TW + 2 * KMAX> &tp, float2 *uv) {
     constexpr int PITCH = TW + 2 * KMAX, RHM = TH + 2 * KMAX;
@@ -159,7 +162,7 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
 #pragma unroll
         for (int j = 0; j < P; ++j)
             if (act[j]) uv[(pk[j] >> 16) * PITCH + (pk[j] & 0xffff)] = make_float2(nu[j], nv[j]);
-        if (a.su && it < k) {  // intermediate state of the output tile
+        if (SOUT && it < k) {  // intermediate state of the output tile
 #pragma unroll
             for (int j = 0; j < P; ++j) {
                 const int gx = gx0 + (pk[j] & 0xffff), gy = gy0 + (pk[j] >> 16);

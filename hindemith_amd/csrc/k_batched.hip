@@ -37,6 +37,8 @@ struct PairDesc {
 
 struct BatchArgs {
     const PairDesc *pairs;
+    const int *tile_pair;  // pair index of every tile of the list
+    unsigned *claims;      // per (phase, XCD) tile-claim counters, kClaimStride apart
     unsigned *barrier;
     int *status;
     long long spin_limit;
@@ -47,14 +49,20 @@ struct SetupTaps {
     Taps dx, dy, dt;
 };
 
-__device__ inline int find_pair(const PairDesc *P, int n, int w) {
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (P[mid].tile_base <= w) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
+// Work-groups are dealt round-robin over the 8 XCDs.  XCD x owns a contiguous
+// share [base, base + cnt) of the tile list, so neighbouring tiles of a pair
+// meet in one L2; inside a phase its work-groups claim tiles from that share
+// with an atomic counter, so cheap (clipped, small) and costly tiles balance.
+constexpr int kXcds = 8;
+// Default barrier polls before a work-group gives up (~190 ns each: ~3 s, far
+// above one phase of any batch that fits in memory).
+constexpr long long kDefaultSpinLimit = 1LL << 24;
+constexpr int kClaimStride = 32;  // counters 128 B apart
+
+__device__ __forceinline__ void xcd_share(int total, int x, int &base, int &cnt) {
+    const int per = total / kXcds, rem = total % kXcds;
+    base = x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per;
+    cnt = per + (x < rem ? 1 : 0);
 }
 
 __device__ __forceinline__ bool grid_sync(unsigned *ctr, unsigned target, int *status,
@@ -92,15 +100,24 @@ __device__ inline float setup_tap(const float *__restrict__ x, int H, int W, int
     return acc;
 }
 
-template <int NT, int TW, int TH, int KMAX, int NTAP>
-__global__ __launch_bounds__(NT) void hs_batched_kernel(BatchArgs b, SetupTaps st, Taps taps) {
+// WPE: waves per SIMD the kernel is compiled to allow (the compiler must fit
+// its VGPRs and SGPRs to it), from which the resident grid is derived.
+template <int NT, int TW, int TH, int KMAX, int NTAP, int WPE>
+__global__ __launch_bounds__(NT, WPE) void hs_batched_kernel(BatchArgs b, SetupTaps st,
+                                                             Taps taps) {
     __shared__ float2 uv[(TW + 2 * KMAX) * (TH + 2 * KMAX)];
+    __shared__ int s_claim;
     const HsTapTable<TW + 2 * KMAX> tp(taps);
     unsigned arrivals = 0;
+    const int xcd = blockIdx.x % kXcds, slot = blockIdx.x / kXcds;
+    const int per_xcd = gridDim.x / kXcds;  // the host launches a multiple of kXcds
+    int base, cnt;
+    xcd_share(b.total_tiles, xcd, base, cnt);
 
     // derivative setup of every tile (Ix = Dx*I1 + Dx*I2, ...), one rounding per op
-    for (int w = blockIdx.x; w < b.total_tiles; w += gridDim.x) {
-        const PairDesc &P = b.pairs[find_pair(b.pairs, b.npairs, w)];
+    for (int i = slot; i < cnt; i += per_xcd) {
+        const int w = base + i;
+        const PairDesc &P = b.pairs[b.tile_pair[w]];
         const int t = w - P.tile_base;
         const int tyi = t / P.tiles_x, txi = t - tyi * P.tiles_x;
         for (int i = threadIdx.x; i < TW * TH; i += NT) {
@@ -124,8 +141,19 @@ __global__ __launch_bounds__(NT) void hs_batched_kernel(BatchArgs b, SetupTaps s
     const int nph = (b.iters + b.k - 1) / b.k;
     for (int ph = 0; ph < nph; ++ph) {
         const int kk = min(b.k, b.iters - ph * b.k);
-        for (int w = blockIdx.x; w < b.total_tiles; w += gridDim.x) {
-            const PairDesc &P = b.pairs[find_pair(b.pairs, b.npairs, w)];
+        unsigned *ctr = b.claims + (ph * kXcds + xcd) * kClaimStride;
+        unsigned next = 0;
+        if (threadIdx.x == 0) next = atomicAdd(ctr, 1u);
+        for (;;) {
+            // hs_tile ends with a barrier, so every thread has read the
+            // previous claim before thread 0 overwrites it
+            if (threadIdx.x == 0) s_claim = (int)min(next, (unsigned)cnt);
+            __syncthreads();
+            const int i = s_claim;
+            if (i >= cnt) break;
+            if (threadIdx.x == 0) next = atomicAdd(ctr, 1u);  // overlaps this tile
+            const int w = base + i;
+            const PairDesc &P = b.pairs[b.tile_pair[w]];
             HsImage im;
             im.Ix = P.Ix;
             im.Iy = P.Iy;
@@ -147,8 +175,9 @@ __global__ __launch_bounds__(NT) void hs_batched_kernel(BatchArgs b, SetupTaps s
     }
 
     if (nph & 1) {  // the last phase wrote (wu, wv)
-        for (int w = blockIdx.x; w < b.total_tiles; w += gridDim.x) {
-            const PairDesc &P = b.pairs[find_pair(b.pairs, b.npairs, w)];
+        for (int i = slot; i < cnt; i += per_xcd) {
+            const int w = base + i;
+            const PairDesc &P = b.pairs[b.tile_pair[w]];
             const int t = w - P.tile_base;
             const int tyi = t / P.tiles_x, txi = t - tyi * P.tiles_x;
             for (int i = threadIdx.x; i < TW * TH; i += NT) {
@@ -165,18 +194,19 @@ __global__ __launch_bounds__(NT) void hs_batched_kernel(BatchArgs b, SetupTaps s
 // ---------------------------------------------------------------------------
 
 struct BVariant {
-    int nt, tw, th, kmax;
+    int nt, tw, th, kmax, wpe;
     void (*kernel[2])(BatchArgs, SetupTaps, Taps);  // 8 taps, any count
 };
 
-#define HB_V(NT, TW, TH, K)                                                               \
-    {NT, TW, TH, K, {hs_batched_kernel<NT, TW, TH, K, 8>, hs_batched_kernel<NT, TW, TH, K, 0>}}
+#define HB_V(NT, TW, TH, K, WPE)                                                          \
+    {NT, TW, TH, K, WPE,                                                                  \
+     {hs_batched_kernel<NT, TW, TH, K, 8, WPE>, hs_batched_kernel<NT, TW, TH, K, 0, WPE>}}
 static const BVariant kBVariants[] = {
-    HB_V(1024, 64, 32, 4),   // 0
-    HB_V(1024, 64, 32, 8),   // 1
-    HB_V(1024, 128, 32, 8),  // 2
-    HB_V(512, 64, 32, 4),    // 3
-    HB_V(1024, 64, 64, 8),   // 4
+    HB_V(1024, 64, 32, 4, 8),   // 0: two work-groups per CU
+    HB_V(1024, 64, 32, 8, 8),   // 1: two work-groups per CU
+    HB_V(1024, 128, 32, 8, 4),  // 2: one
+    HB_V(512, 64, 32, 4, 4),    // 3: two
+    HB_V(1024, 64, 64, 8, 4),   // 4: one
 };
 #undef HB_V
 static constexpr int kNumBVariants = sizeof(kBVariants) / sizeof(kBVariants[0]);
@@ -187,8 +217,9 @@ struct BatchPlan {
     std::vector<char> key;
     int device = 0;
     int variant = 0, k = 1, grid = 0, total_tiles = 0;
-    PairDesc *d_pairs = nullptr;
-    unsigned *d_sync = nullptr;  // [0] barrier counter, [1] status
+    PairDesc *d_pairs = nullptr;  // npairs descriptors, then total_tiles pair indices
+    unsigned *d_sync = nullptr;  // [0] barrier counter, [1] status, claims from [64]
+    int64_t sync_words = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -214,10 +245,12 @@ static void append(std::vector<char> &k, const T &v) {
 
 static int launch_batch(const BatchPlan &P, const SetupTaps &st, const Taps &av, int npairs,
                         int iters, long long spin_limit, hipStream_t s) {
-    int rc = launch_fill_u32(P.d_sync, 0u, 2, s);
+    int rc = launch_fill_u32(P.d_sync, 0u, P.sync_words, s);
     if (rc) return rc;
     BatchArgs a;
     a.pairs = P.d_pairs;
+    a.tile_pair = reinterpret_cast<const int *>(P.d_pairs + npairs);
+    a.claims = P.d_sync + 64;
     a.barrier = P.d_sync;
     a.status = (int *)(P.d_sync + 1);
     a.spin_limit = spin_limit;
@@ -231,14 +264,40 @@ static int launch_batch(const BatchPlan &P, const SetupTaps &st, const Taps &av,
     return HMX_OK;
 }
 
-static int resident_grid(int variant, int nd, int total_tiles, int device, int &grid) {
+// Work-groups of the variant that can be resident on one CU: the smaller of
+// the runtime's occupancy figure and a bound from the kernel's own resources
+// (the waves per SIMD it was compiled for, 512 VGPRs per lane per SIMD in
+// granules of 8, the CU's LDS; one work-group if it uses scratch).  Every work-group of the grid must
+// be resident at once, or the grid barrier waits until its spin limit.
+static int resident_per_cu(int variant, int nd, int device, int &api, int &own, int &cus) {
     const BVariant &V = kBVariants[variant];
-    int per_cu = 0, cus = 0;
-    HMX_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, reinterpret_cast<const void *>(V.kernel[nd == 8 ? 0 : 1]), V.nt, 0));
+    const void *kern = reinterpret_cast<const void *>(V.kernel[nd == 8 ? 0 : 1]);
+    HMX_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&api, kern, V.nt, 0));
+    hipFuncAttributes fa;
+    HMX_HIP_TRY(hipFuncGetAttributes(&fa, kern));
+    int lds = 0;
+    HMX_HIP_TRY(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                                      device));
     HMX_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
-    if (per_cu < 1) return fail(HMX_ERR_UNSUPPORTED, "batched kernel cannot be resident");
-    grid = std::min(total_tiles, per_cu * cus);
+    const int waves_per_simd_per_block = ((V.nt + 63) / 64 + 3) / 4;
+    const int vgpr = std::max(8, (fa.numRegs + 7) / 8 * 8);
+    // the declared waves per SIMD bound every register kind (SGPRs included,
+    // which the runtime's figure does not see); registers can only lower it
+    const int waves_per_simd = std::min(V.wpe, 512 / vgpr);
+    own = waves_per_simd / waves_per_simd_per_block;
+    if (fa.sharedSizeBytes > 0 && lds > 0)
+        own = std::min(own, (int)(lds / fa.sharedSizeBytes));
+    if (fa.localSizeBytes > 0) own = std::min(own, 1);
+    return HMX_OK;
+}
+
+static int resident_grid(int variant, int nd, int total_tiles, int device, int &grid) {
+    int api = 0, own = 0, cus = 0;
+    int rc = resident_per_cu(variant, nd, device, api, own, cus);
+    if (rc) return rc;
+    const int cap = std::min(api, own) * cus / kXcds * kXcds;  // a multiple of the XCD count
+    if (cap < kXcds) return fail(HMX_ERR_UNSUPPORTED, "batched kernel cannot be resident");
+    grid = std::min(cap, (total_tiles + kXcds - 1) / kXcds * kXcds);
     return HMX_OK;
 }
 
@@ -247,6 +306,20 @@ static int resident_grid(int variant, int nd, int total_tiles, int device, int &
 using namespace hmx;
 
 extern "C" int hmx_hs_batched_variant_count(void) { return kNumBVariants; }
+
+extern "C" int hmx_hs_batched_resident(int variant, int ntaps, int *api_per_cu,
+                                       int *own_per_cu, int *cus) {
+    HMX_REQUIRE(variant >= 0 && variant < kNumBVariants, "unknown batched variant");
+    int device = 0;
+    HMX_HIP_TRY(hipGetDevice(&device));
+    int a = 0, o = 0, c = 0;
+    int rc = resident_per_cu(variant, ntaps, device, a, o, c);
+    if (rc) return rc;
+    if (api_per_cu) *api_per_cu = a;
+    if (own_per_cu) *own_per_cu = o;
+    if (cus) *cus = c;
+    return HMX_OK;
+}
 
 extern "C" int hmx_hs_batched_variant_info(int variant, int *tile_w, int *tile_h, int *threads,
                                            int *kmax) {
@@ -342,9 +415,17 @@ static int batched_impl(const hmx_pair *pairs, int npairs, const SetupTaps &st, 
         P.total_tiles = tiles;
         int rc = resident_grid(variant, av.nd, tiles, device, P.grid);
         if (rc) return rc;
-        HMX_HIP_TRY(hipMalloc(&P.d_pairs, sizeof(PairDesc) * npairs));
-        HMX_HIP_TRY(hipMalloc(&P.d_sync, 256));
+        std::vector<int> tile_pair(tiles);
+        for (int i = 0; i < npairs; ++i)
+            std::fill(tile_pair.begin() + desc[i].tile_base,
+                      tile_pair.begin() + desc[i].tile_base + desc[i].ntiles, i);
+        HMX_HIP_TRY(hipMalloc(&P.d_pairs, sizeof(PairDesc) * npairs + sizeof(int) * tiles));
+        const int64_t phases = (iters + k - 1) / k;
+        P.sync_words = 64 + (phases + 1) * kXcds * kClaimStride;
+        HMX_HIP_TRY(hipMalloc(&P.d_sync, sizeof(unsigned) * P.sync_words));
         HMX_HIP_TRY(hipMemcpy(P.d_pairs, desc.data(), sizeof(PairDesc) * npairs,
+                              hipMemcpyHostToDevice));
+        HMX_HIP_TRY(hipMemcpy(P.d_pairs + npairs, tile_pair.data(), sizeof(int) * tiles,
                               hipMemcpyHostToDevice));
         if (flags & HMX_BATCH_GRAPH) {
             hipStream_t cs;
@@ -403,7 +484,7 @@ extern "C" int hmx_hs_batched(const hmx_pair *pairs, int npairs, int ndx, const 
     int rc = make_batch_taps(st, av, ndx, wx, oxx, oyx, ndy, wy, oxy, oyy, ndt, wt, oxt, oyt,
                              nda, wa, oxa, oya);
     if (rc) return rc;
-    if (spin_limit <= 0) spin_limit = 100000000LL;
+    if (spin_limit <= 0) spin_limit = kDefaultSpinLimit;
     return batched_impl(pairs, npairs, st, av, iters, variant, k, flags, spin_limit,
                         as_stream(stream));
 }
@@ -427,10 +508,15 @@ extern "C" int hmx_hs_batched_tune(const hmx_pair *pairs, int npairs, int ndx, c
     int bv = 1, bk = 8;
     for (int vi = 0; vi < kNumBVariants && !rc; ++vi) {
         for (int k = 2; k <= kBVariants[vi].kmax && !rc; k *= 2) {
-            rc = batched_impl(pairs, npairs, st, av, iters, vi, k, HMX_BATCH_SYNC, 100000000LL, s);
+            rc = batched_impl(pairs, npairs, st, av, iters, vi, k, HMX_BATCH_SYNC,
+                              kDefaultSpinLimit, s);
+            if (rc == HMX_ERR_TIMEOUT) {  // this grid did not stay resident: skip it
+                rc = HMX_OK;
+                continue;
+            }
             if (rc) break;
             HMX_HIP_TRY(hipEventRecord(e0, s));
-            rc = batched_impl(pairs, npairs, st, av, iters, vi, k, 0, 100000000LL, s);
+            rc = batched_impl(pairs, npairs, st, av, iters, vi, k, 0, kDefaultSpinLimit, s);
             if (rc) break;
             HMX_HIP_TRY(hipEventRecord(e1, s));
             HMX_HIP_TRY(hipEventSynchronize(e1));

@@ -21,6 +21,7 @@
 // precomputed tap deltas.
 #include "hs_tile.h"
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <utility>
@@ -42,7 +43,7 @@ struct HsArgs {
     int64_t sstride;
 };
 
-template <int NT, int TW, int TH, int KMAX, int NTAP>
+template <int NT, int TW, int TH, int KMAX, int NTAP, bool SOUT = false>
 __global__ __launch_bounds__(NT) void hs_tb_kernel(HsArgs a, Taps taps) {
     __shared__ float2 uv[(TW + 2 * KMAX) * (TH + 2 * KMAX)];
     const int64_t off = (int64_t)blockIdx.y * a.bstride;  // pair of a batch
@@ -59,11 +60,13 @@ __global__ __launch_bounds__(NT) void hs_tb_kernel(HsArgs a, Taps taps) {
     im.H = a.H;
     im.W = a.W;
     im.tiles_x = a.tiles_x;
-    im.su = a.su ? a.su + off : nullptr;
-    im.sv = a.sv ? a.sv + off : nullptr;
-    im.sstride = a.sstride;
+    if (SOUT) {
+        im.su = a.su + off;
+        im.sv = a.sv + off;
+        im.sstride = a.sstride;
+    }
     const HsTapTable<TW + 2 * KMAX> tp(taps);
-    hs_tile<NT, TW, TH, KMAX, NTAP>(im, xcd_remap(blockIdx.x, a.ntiles), a.k, tp, uv);
+    hs_tile<NT, TW, TH, KMAX, NTAP, SOUT>(im, xcd_remap(blockIdx.x, a.ntiles), a.k, tp, uv);
 }
 
 template <int NT, int TW, int TH, int KMAX, int S, int NTAP>
@@ -92,6 +95,7 @@ __global__ __launch_bounds__(NT) void hs_strip_kernel(HsArgs a, Taps taps) {
 // 8*(TW+2K)(TH+2K) bytes.
 struct HsVariant {
     int nt, tw, th, kmax;
+    int strip;                        // rows per thread per column (0: rectangular tile)
     void (*kernel[5])(HsArgs, Taps);  // by tap class: 4, 5, 8, 9 taps, any
 };
 
@@ -107,13 +111,13 @@ static int tap_class(int nd) {
 }
 
 #define HS_V(NT, TW, TH, K)                                                             \
-    {NT, TW, TH, K,                                                                     \
+    {NT, TW, TH, K, 0,                                                                  \
      {hs_tb_kernel<NT, TW, TH, K, 4>, hs_tb_kernel<NT, TW, TH, K, 5>,                  \
       hs_tb_kernel<NT, TW, TH, K, 8>, hs_tb_kernel<NT, TW, TH, K, 9>,                  \
       hs_tb_kernel<NT, TW, TH, K, 0>}}
 // column-strip variants (S pixels per thread per column)
 #define HS_S(NT, TW, TH, K, S)                                                          \
-    {NT, TW, TH, K,                                                                     \
+    {NT, TW, TH, K, S,                                                                  \
      {hs_strip_kernel<NT, TW, TH, K, S, 4>, hs_strip_kernel<NT, TW, TH, K, S, 5>,      \
       hs_strip_kernel<NT, TW, TH, K, S, 8>, hs_strip_kernel<NT, TW, TH, K, S, 9>,      \
       hs_strip_kernel<NT, TW, TH, K, S, 0>}}
@@ -176,8 +180,18 @@ int launch_hs_tb(int variant, const float *Ix, const float *Iy, const float *It,
     a.tiles_x = (W + V.tw - 1) / V.tw;
     const int tiles_y = (H + V.th - 1) / V.th;
     a.ntiles = a.tiles_x * tiles_y;
-    hipLaunchKernelGGL(V.kernel[tap_class(taps.nd)], dim3(a.ntiles, B), dim3(V.nt), 0, s, a,
-                       taps);
+    auto kern = V.kernel[tap_class(taps.nd)];
+    if (su) {
+        // state output exists only for the 1024-thread 64x32 k<=8 shape
+        static void (*const kStateKernels[5])(HsArgs, Taps) = {
+            hs_tb_kernel<1024, 64, 32, 8, 4, true>, hs_tb_kernel<1024, 64, 32, 8, 5, true>,
+            hs_tb_kernel<1024, 64, 32, 8, 8, true>, hs_tb_kernel<1024, 64, 32, 8, 9, true>,
+            hs_tb_kernel<1024, 64, 32, 8, 0, true>};
+        if (V.strip || V.nt != 1024 || V.tw != 64 || V.th != 32 || V.kmax != 8)
+            return fail(HMX_ERR_UNSUPPORTED, "hs_tb: state output needs the 1024x64x32x8 variant");
+        kern = kStateKernels[tap_class(taps.nd)];
+    }
+    hipLaunchKernelGGL(kern, dim3(a.ntiles, B), dim3(V.nt), 0, s, a, taps);
     HMX_LAUNCH_CHECK("hs_tb");
     return HMX_OK;
 }
@@ -261,6 +275,11 @@ extern "C" int hmx_hs_variant_info(int variant, int *tile_w, int *tile_h, int *t
     return HMX_OK;
 }
 
+extern "C" int hmx_hs_variant_strip(int variant) {
+    HMX_REQUIRE(variant >= 0 && variant < kNumVariants, "unknown HS kernel variant");
+    return kVariants[variant].strip;
+}
+
 extern "C" int hmx_hs_jacobi_cfg(const float *Ix, const float *Iy, const float *It,
                                  const float *den, float alpha2, int H, int W, int nda,
                                  const float *wa, const int *oxa, const int *oya, int iters,
@@ -304,30 +323,52 @@ extern "C" int hmx_hs_tune(const float *Ix, const float *Iy, const float *It, fl
     hipEvent_t e0, e1;
     HMX_HIP_TRY(hipEventCreate(&e0));
     HMX_HIP_TRY(hipEventCreate(&e1));
-    const int probe_iters = 48;  // divisible by every k tried
-    float best = 1e30f;
-    int best_v = 0, best_k = 1;
-    for (int vi = 0; vi < kNumVariants; ++vi) {
-        for (int k = 1; k <= kVariants[vi].kmax; k *= 2) {
-            // warm-up, then timed
-            rc = hs_jacobi_run(Ix, Iy, It, nullptr, alpha2, H, W, t, probe_iters, u, v, work_u,
-                               work_v, vi, k, s);
-            if (rc) break;
-            HMX_HIP_TRY(hipEventRecord(e0, s));
-            rc = hs_jacobi_run(Ix, Iy, It, nullptr, alpha2, H, W, t, probe_iters, u, v, work_u,
-                               work_v, vi, k, s);
-            if (rc) break;
-            HMX_HIP_TRY(hipEventRecord(e1, s));
-            HMX_HIP_TRY(hipEventSynchronize(e1));
+    // min over reps of one timed run of iters iterations (after one warm-up run)
+    auto time_cfg = [&](int vi, int k, int iters, int reps, float &best_ms) -> int {
+        best_ms = 1e30f;
+        int r = hs_jacobi_run(Ix, Iy, It, nullptr, alpha2, H, W, t, iters, u, v, work_u, work_v,
+                              vi, k, s);
+        for (int i = 0; i < reps && !r; ++i) {
+            if (hipEventRecord(e0, s) != hipSuccess) return fail(HMX_ERR_HIP, "tune: event");
+            r = hs_jacobi_run(Ix, Iy, It, nullptr, alpha2, H, W, t, iters, u, v, work_u, work_v,
+                              vi, k, s);
+            if (r) break;
             float ms = 0.0f;
-            HMX_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-            if (ms < best) {
+            if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+                return fail(HMX_ERR_HIP, "tune: event timing");
+            best_ms = std::min(best_ms, ms);
+        }
+        return r;
+    };
+    // pass 1: every (variant, k) on a short run; pass 2: the four fastest on a
+    // run five times longer, so that near-equal candidates are told apart by
+    // more than launch noise (both iteration counts divide by every k tried)
+    struct Cand {
+        float ms;
+        int v, k;
+    };
+    std::vector<Cand> cands;
+    for (int vi = 0; vi < kNumVariants && !rc; ++vi)
+        for (int k = 1; k <= kVariants[vi].kmax && !rc; k *= 2) {
+            float ms;
+            rc = time_cfg(vi, k, 48, 2, ms);
+            cands.push_back({ms, vi, k});
+        }
+    int best_v = 0, best_k = 1;
+    if (!rc) {
+        std::sort(cands.begin(), cands.end(),
+                  [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
+        float best = 1e30f;
+        for (size_t i = 0; i < cands.size() && i < 4 && !rc; ++i) {
+            float ms;
+            rc = time_cfg(cands[i].v, cands[i].k, 240, 3, ms);
+            if (!rc && ms < best) {
                 best = ms;
-                best_v = vi;
-                best_k = k;
+                best_v = cands[i].v;
+                best_k = cands[i].k;
             }
         }
-        if (rc) break;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

@@ -196,7 +196,10 @@ def hs_variants():
         tw, th, nt, km = (ctypes.c_int() for _ in range(4))
         check(L.hmx_hs_variant_info(i, ctypes.byref(tw), ctypes.byref(th), ctypes.byref(nt),
                                     ctypes.byref(km)), "variant_info")
-        out.append(dict(tile_w=tw.value, tile_h=th.value, threads=nt.value, kmax=km.value))
+        strip = L.hmx_hs_variant_strip(i)
+        out.append(dict(tile_w=tw.value, tile_h=th.value, threads=nt.value, kmax=km.value,
+                        strip=strip,
+                        kernel="hs_strip_kernel" if strip > 0 else "hs_tb_kernel"))
     return out
 
 

@@ -185,6 +185,9 @@ int hmx_hs_jacobi_cfg(const float *Ix, const float *Iy, const float *It, const f
 int hmx_hs_variant_count(void);
 /* Tile width/height, threads and max k of a variant. */
 int hmx_hs_variant_info(int variant, int *tile_w, int *tile_h, int *threads, int *kmax);
+/* Rows per thread per column of a column-strip variant (hs_strip_kernel), 0 for a
+ * rectangular-tile variant (hs_tb_kernel), or a negative HMX_ERR_* code. */
+int hmx_hs_variant_strip(int variant);
 /* Time every (variant, k) candidate on the given buffers (their contents are
  * overwritten) and cache the best for this H x W; returns the choice. */
 int hmx_hs_tune(const float *Ix, const float *Iy, const float *It, float alpha2, int H, int W,
@@ -267,9 +270,11 @@ struct hmx_pair {
 
 /* Setup + iters Jacobi iterations of every pair in ONE persistent launch: a
  * grid no larger than the GPU holds at once walks the tiles of all pairs,
- * with a grid-wide barrier between phases of k iterations.  variant = -1
- * uses the autotuned (or default) tile shape and k.  Barrier waits give up
- * after spin_limit polls (<= 0: default) and then every work-group exits;
+ * with a grid-wide barrier between phases of k iterations.  Each of the 8
+ * XCDs owns a contiguous share of the tile list; its work-groups claim tiles
+ * from that share through a per-phase counter.  variant = -1 uses the
+ * autotuned (or default) tile shape and k.  Barrier waits give up after
+ * spin_limit polls (<= 0: 2^24, about 3 s) and then every work-group exits;
  * with HMX_BATCH_SYNC the call returns HMX_ERR_TIMEOUT in that case.
  * Launch state (descriptors, barrier words, graph) is cached per signature;
  * calls that overlap on one stream must not share pairs. */
@@ -290,6 +295,12 @@ int hmx_hs_batched_tune(const struct hmx_pair *pairs, int npairs,
                         int iters, void *stream, int *variant, int *k);
 int hmx_hs_batched_variant_count(void);
 int hmx_hs_batched_variant_info(int variant, int *tile_w, int *tile_h, int *threads, int *kmax);
+/* Work-groups of a batched variant (averaging stencil of ntaps taps) resident per CU:
+ * the runtime's occupancy figure, the bound from the kernel's registers and LDS, and
+ * the CU count.  The persistent grid is min(api, own) x cus rounded down to a
+ * multiple of 8 (the XCD count). */
+int hmx_hs_batched_resident(int variant, int ntaps, int *api_per_cu, int *own_per_cu,
+                            int *cus);
 /* Free every cached batched launch (descriptors, graphs). */
 int hmx_hs_batched_release(void);
 

@@ -98,3 +98,19 @@ def test_batched_equals_per_pair_plain_solves():
     torch.cuda.synchronize()
     for (bu, bv), (pu, pv) in zip(batched, plain):
         assert torch.equal(bu, pu) and torch.equal(bv, pv)
+
+
+def test_full_size_batch_every_variant_stays_resident():
+    """64 pairs with sides U(64, 1024): every variant's grid is the full
+    resident size, so a grid larger than what the GPU holds at once would end
+    in a barrier time-out here.  Results must equal the per-pair plain path."""
+    from hindemith_amd.batched import batched_variants
+    pairs, B = make(64, seed=0, side=(64, 1024), iters=9)
+    plain = [(u.clone(), v.clone()) for u, v in B.solve_each()]
+    for vi, info in enumerate(batched_variants()):
+        assert 1 <= min(info["api_per_cu"], info["own_per_cu"]), info
+        for k in sorted({2, info["kmax"]}):
+            out = B.run(graph=False, sync=True, spin_limit=1 << 22, variant=vi, k=k)
+            torch.cuda.synchronize()
+            for i, ((bu, bv), (pu, pv)) in enumerate(zip(out, plain)):
+                assert torch.equal(bu, pu) and torch.equal(bv, pv), (vi, k, i)

@@ -127,6 +127,37 @@ def _time_ms(fn, reps, warm=2):
     return e0.elapsed_time(e1) / reps
 
 
+def _time_graph_ms(fn, launches=50, reps=5):
+    """Per-call GPU time of fn: launches calls captured in one hipGraph, the
+    graph replayed reps times, median of replay time / launches.  Removes the
+    host-side cost of each call, which exceeds the kernel time of a small
+    bandwidth kernel."""
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(g, stream=s):
+            for _ in range(launches):
+                fn()
+    torch.cuda.current_stream().wait_stream(s)
+    g.replay()
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        g.replay()
+        e1.record()
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1) / launches)
+    del g
+    return float(np.median(times))
+
+
 def extra_4k(K, S, taps):
     """3840x2160 fp32, 500 iterations, affine flow (same kernel path)."""
     import torch
@@ -151,7 +182,8 @@ def extra_4k(K, S, taps):
 
 
 def extra_bandwidth(K, S):
-    """2048x2048 bandwidth kernels, 200 repeats each (GB/s of algorithmic bytes)."""
+    """2048x2048 bandwidth kernels, 200 repeats each (GB/s of algorithmic
+    bytes), timed as hipGraph replays of 50 launches."""
     import torch
     H = W = 2048
     N = H * W
@@ -160,18 +192,18 @@ def extra_bandwidth(K, S):
     y = torch.empty_like(x)
     out = {}
     lap = S.laplacian5_taps()
-    ms = _time_ms(lambda: K.stencil2d(x, lap, out=y), 200)
+    ms = _time_graph_ms(lambda: K.stencil2d(x, lap, out=y))
     out["stencil_5pt"] = {"us": round(ms * 1e3, 2), "GBps": round(8 * N / ms / 1e6, 1)}
     planes, ox, oy = S.laplacian5_dia_planes(H, W)
     A = torch.from_numpy(planes).cuda()
-    ms = _time_ms(lambda: K.dia2d_spmv(A, ox, oy, x, out=y), 200)
+    ms = _time_graph_ms(lambda: K.dia2d_spmv(A, ox, oy, x, out=y))
     out["dia2d_5diag"] = {"us": round(ms * 1e3, 2), "GBps": round(28 * N / ms / 1e6, 1)}
     ip, ix, d = S.dia2d_to_csr(planes, ox, oy)
     tip, tix, td = (torch.from_numpy(a).cuda() for a in (ip, ix, d))
     xf = x.reshape(-1)
     yf = y.reshape(-1)
     nnz = int(ip[-1])
-    ms = _time_ms(lambda: K.csr_spmv(tip, tix, td, xf, out=yf), 200)
+    ms = _time_graph_ms(lambda: K.csr_spmv(tip, tix, td, xf, out=yf))
     out["csr_5pt"] = {"nnz": nnz, "us": round(ms * 1e3, 2),
                       "GBps": round((8 * nnz + 12 * N) / ms / 1e6, 1)}
     del tip, tix, td
@@ -179,8 +211,8 @@ def extra_bandwidth(K, S):
     tip, tix, td = (torch.from_numpy(a).cuda() for a in (ip, ix, d))
     nnz = int(ip[-1])
     bins = K.CSRBins(ip, "cuda")
-    ms = _time_ms(lambda: K.csr_spmv_binned(tip, tix, td, xf, bins, out=yf), 200)
-    ms_vec = _time_ms(lambda: K.csr_spmv(tip, tix, td, xf, out=yf), 50)
+    ms = _time_graph_ms(lambda: K.csr_spmv_binned(tip, tix, td, xf, bins, out=yf))
+    ms_vec = _time_graph_ms(lambda: K.csr_spmv(tip, tix, td, xf, out=yf), launches=20)
     out["csr_powerlaw_zipf1.5"] = {"nnz": nnz, "us": round(ms * 1e3, 2),
                                    "GBps": round((8 * nnz + 12 * N) / ms / 1e6, 1),
                                    "us_unbinned": round(ms_vec * 1e3, 2)}
@@ -189,10 +221,11 @@ def extra_bandwidth(K, S):
 
 def extra_batched(S):
     """64 uneven pairs (sides U(64,1024)), 200 iterations: one persistent
-    graph-captured launch vs 64 plain per-pair solves.  Two plain baselines:
-    the temporally blocked solve of each pair (speedup), and each pair with one
-    fused Jacobi launch per iteration, the reference's launch structure
-    (speedup_vs_launch_per_iteration)."""
+    graph-captured launch vs 64 plain per-pair solves.  Plain baselines: the
+    temporally blocked solve of each pair called eagerly (speedup) and the same
+    calls captured in one hipGraph (speedup_vs_plain_graph, GPU time only), and
+    each pair with one fused Jacobi launch per iteration, the reference's launch
+    structure (speedup_vs_launch_per_iteration)."""
     import torch
     from hindemith_amd.batched import BatchedHS
     pairs = S.batched_pairs(64, seed=0)
@@ -203,13 +236,16 @@ def extra_batched(S):
     B.run(graph=True, sync=True)
     ms_b = _time_ms(lambda: B.run(graph=True), 5)
     ms_p = _time_ms(B.solve_each, 3, warm=1)
+    # the same 64 solves captured in one hipGraph: GPU time without host cost
+    ms_pg = _time_graph_ms(B.solve_each, launches=1, reps=3)
     ms_1 = _time_ms(lambda: B.solve_each(variant=0, k=1), 2, warm=1)
     px = sum(p["I1"].size for p in pairs)
     return {"pairs": 64, "megapixels": round(px / 1e6, 2), "iters": 200,
             "batched_pairs_per_s": round(64 / (ms_b * 1e-3), 1),
             "plain_pairs_per_s": round(64 / (ms_p * 1e-3), 1),
             "speedup": round(ms_p / ms_b, 2), "batched_ms": round(ms_b, 3),
-            "plain_ms": round(ms_p, 3),
+            "plain_ms": round(ms_p, 3), "plain_graph_ms": round(ms_pg, 3),
+            "speedup_vs_plain_graph": round(ms_pg / ms_b, 2),
             "plain_launch_per_iteration_ms": round(ms_1, 3),
             "speedup_vs_launch_per_iteration": round(ms_1 / ms_b, 2),
             "batched_mpx_iter_per_s": round(px * 200 / (ms_b * 1e-3) / 1e6, 1),

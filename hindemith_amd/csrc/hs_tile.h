@@ -16,10 +16,12 @@ struct HsImage {
     float *u_out, *v_out;
     float alpha2;
     int H, W, tiles_x;
-    // Optional: after step j (1..k) the output tile's state is also written to
-    // (su, sv) + (j-1)*sstride (used to recompute checkpoint segments).
-    float *su = nullptr, *sv = nullptr;
-    int64_t sstride = 0;
+    // Optional (kernels built with LOUT): for each step j (1..k), the values
+    // the step linearises about on the output tile -- t and (ubar, vbar) of
+    // the state before the step -- are written to (lt, lub, lvb) +
+    // (j-1)*lstride (the adjoint's recompute of a checkpoint segment).
+    float *lt = nullptr, *lub = nullptr, *lvb = nullptr;
+    int64_t lstride = 0;
 };
 
 // Largest averaging stencil the fused kernel takes: any radius-1 stencil with
@@ -50,8 +52,8 @@ struct HsTapTable {
 
 // The per-pixel update of the canonical program, one rounding per op:
 //   t = ((Ix*ubar + Iy*vbar) + It) / den;  u' = ubar - Ix*t;  v' = vbar - Iy*t
-__device__ __forceinline__ void hs_update(float ix, float iy, float it, float den, float ub,
-                                          float vb, float &un, float &vn) {
+__device__ __forceinline__ float hs_update(float ix, float iy, float it, float den, float ub,
+                                           float vb, float &un, float &vn) {
     const float m1 = ix * ub;
     const float m2 = iy * vb;
     const float s1 = m1 + m2;
@@ -61,6 +63,7 @@ __device__ __forceinline__ void hs_update(float ix, float iy, float it, float de
     const float p2 = iy * tt;
     un = ub - p1;
     vn = vb - p2;
+    return tt;
 }
 
 // Advance output tile t of image a by k (<= KMAX) Jacobi steps.  uv is the
@@ -68,12 +71,14 @@ __device__ __forceinline__ void hs_update(float ix, float iy, float it, float de
 // the fixed row pitch TW + 2*KMAX so every tap is a constant LDS delta.
 // NTAP > 0: the stencil has exactly NTAP taps (straight-line code, batched LDS
 // reads); NTAP == 0: any count up to HS_MAX_TAPS, tested per tap.
-// SOUT: also write the intermediate states 1..k-1 of the output tile to
-// a.su/a.sv (a compile-time switch: the stores cost registers in every
-// kernel that carries them).
-template <int NT, int TW, int TH, int KMAX, int NTAP, bool SOUT = false>
+// LOUT: also write each step's (t, ubar, vbar) of the output tile to a.lt,
+// a.lub, a.lvb, staged through lin (3 x TW x TH floats of LDS) and stored in
+// tile order between the step's two barriers (a compile-time switch: the
+// extra work and LDS stay out of the kernels that do not need it).
+template <int NT, int TW, int TH, int KMAX, int NTAP, bool LOUT = false>
 __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
-                                        const HsTapTable<TW + 2 * KMAX> &tp, float2 *uv) {
+                                        const HsTapTable<TW + 2 * KMAX> &tp, float2 *uv,
+                                        float *lin = nullptr) {
     constexpr int PITCH = TW + 2 * KMAX, RHM = TH + 2 * KMAX;
     constexpr int NPX = PITCH * RHM;
     constexpr int P = (NPX + NT - 1) / NT;
@@ -155,21 +160,33 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
                         }
                     }
                 }
-                hs_update(cx[j], cy[j], ct[j], cd[j], ub, vb, nu[j], nv[j]);
+                const float tt = hs_update(cx[j], cy[j], ct[j], cd[j], ub, vb, nu[j], nv[j]);
+                if (LOUT) {
+                    const int lx = gx0 + rx - x0, ly = gy0 + ry - y0;
+                    if (lx >= 0 && lx < TW && ly >= 0 && ly < TH) {
+                        const int li = ly * TW + lx;
+                        lin[li] = tt;
+                        lin[TW * TH + li] = ub;
+                        lin[2 * TW * TH + li] = vb;
+                    }
+                }
             }
         }
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < P; ++j)
             if (act[j]) uv[(pk[j] >> 16) * PITCH + (pk[j] & 0xffff)] = make_float2(nu[j], nv[j]);
-        if (SOUT && it < k) {  // intermediate state of the output tile
-#pragma unroll
-            for (int j = 0; j < P; ++j) {
-                const int gx = gx0 + (pk[j] & 0xffff), gy = gy0 + (pk[j] >> 16);
-                if (act[j] && gx >= x0 && gx < x1 && gy >= y0 && gy < y1) {
-                    const int64_t g = (int64_t)gy * a.W + gx + (int64_t)(it - 1) * a.sstride;
-                    a.su[g] = nu[j];
-                    a.sv[g] = nv[j];
+        if (LOUT) {
+            // lin is rewritten only in the next step's compute, after the
+            // barrier below
+            const int64_t so = (int64_t)(it - 1) * a.lstride;
+            for (int i = tid; i < TW * TH; i += NT) {
+                const int gy = y0 + i / TW, gx = x0 + i % TW;
+                if (gx < x1 && gy < y1) {
+                    const int64_t g = so + (int64_t)gy * a.W + gx;
+                    a.lt[g] = lin[i];
+                    a.lub[g] = lin[TW * TH + i];
+                    a.lvb[g] = lin[2 * TW * TH + i];
                 }
             }
         }
@@ -299,7 +316,7 @@ __device__ __forceinline__ void hs_tile_strip(const HsImage &a, int t, int k,
             }
 #pragma unroll
             for (int s = 0; s < S; ++s)
-                hs_update(cx[m][s], cy[m][s], ct[m][s], cd[m][s], ub[s], vb[s], nu[m][s],
+                (void)hs_update(cx[m][s], cy[m][s], ct[m][s], cd[m][s], ub[s], vb[s], nu[m][s],
                           nv[m][s]);
         }
         __syncthreads();

@@ -19,9 +19,10 @@
 // so the transpose is a gather over at most (|o|+1)^2 sources per tap.
 //
 // Memory: states are checkpointed every C iterations in the forward pass; the
-// reverse pass recomputes one segment of C states at a time (with the same
-// fused kernel, so the recomputed states are bit-identical), then walks it
-// backwards.
+// reverse pass recomputes one segment of C steps at a time with the same fused
+// kernel (bit-identical states), keeping for each step only what its reverse
+// step needs on the pixel itself -- t and (ubar, vbar) -- then walks the
+// segment backwards.
 #include "hmx_common.h"
 
 #include <algorithm>
@@ -35,8 +36,8 @@ namespace hmx {
 int launch_hs_tb(int variant, const float *Ix, const float *Iy, const float *It,
                  const float *den, float alpha2, int H, int W, const Taps &taps, int k,
                  const float *u_in, const float *v_in, float *u_out, float *v_out,
-                 hipStream_t s, int B, const float *alpha2_b, float *su, float *sv,
-                 int64_t sstride);
+                 hipStream_t s, int B, const float *alpha2_b, float *lt, float *lub,
+                 float *lvb, int64_t lstride);
 int hs_jacobi_run(const float *Ix, const float *Iy, const float *It, const float *den,
                   float alpha2, int H, int W, const Taps &taps, int iters, float *u, float *v,
                   float *wu, float *wv, int variant, int k, hipStream_t s, int B,
@@ -80,15 +81,16 @@ __device__ inline float tap_apply_T(const float *__restrict__ g, int H, int W, i
 // steps for an output tile.  The gradient (gu, gv) lives in LDS on the tile
 // grown by m (clipped to the image) and shrinks by one pixel per step, exactly
 // like the forward kernel's region; the accumulators gIx, gIy, gIt of the
-// tile's own pixels stay in registers and are added to memory once per
-// launch; each step stages only its state x_i on the tile grown by one.
-// HBM bytes per pixel-step drop from ~60 (one step per launch) to ~24 (m=4).
+// tile's own pixels stay in registers and are stored once per launch; each
+// step stages only the recomputed (t, ubar, vbar) of its forward step on the
+// tile itself.
 
 struct AdjTbArgs {
     const float *Ix, *Iy, *It;
     const float *gu_in, *gv_in;  // dL/dx_hi
     float *gu_out, *gv_out;      // dL/dx_{hi-m}
-    const float *xu[8], *xv[8];  // step j (1..m) uses state x_{hi-j}: xu[j-1], xv[j-1]
+    // step j (1..m) reverses forward step hi-j, linearised about lt/lub/lvb[j-1]
+    const float *lt[8], *lub[8], *lvb[8];
     float *gIx, *gIy, *gIt;
     double *ga2;                 // [pair][tile] running sums of -s t
     const float *alpha2_b;
@@ -98,15 +100,25 @@ struct AdjTbArgs {
 
 constexpr int ADJ_TW = 64, ADJ_TH = 32, ADJ_NT = 1024, ADJ_MMAX = 4;
 
+// Compiled for 8 waves per SIMD: two 1024-thread work-groups per CU.
+// Thread layout: slots 0..PT-1 of a thread are tile pixels in tile order
+// (i = tid + j*NT), so the per-pixel accumulators and the step's (t, ubar,
+// vbar) exist only there and load coalesced; the remaining PH slots cover the
+// halo ring of the region.  The gradient g of every slot stays in registers
+// (only its own thread reads or writes it); only (g_ubar, g_vbar), which
+// pass B gathers from neighbours, lives in LDS, double-buffered, so pass B of
+// step s and pass A of step s+1 (same region) run back to back with one
+// barrier per step.
 template <int NT, int TW, int TH, int MMAX>
-__global__ __launch_bounds__(NT) void hs_adj_tb_kernel(AdjTbArgs a, Taps taps) {
+__global__ __launch_bounds__(NT, 8) void hs_adj_tb_kernel(AdjTbArgs a, Taps taps) {
     constexpr int PITCH = TW + 2 * MMAX, RHM = TH + 2 * MMAX, NPX = PITCH * RHM;
-    constexpr int P = (NPX + NT - 1) / NT;
-    constexpr int XW = TW + 2, XH = TH + 2;
-    __shared__ float2 G[NPX];    // gradient wrt the state, on the current region
-    __shared__ float2 GB[NPX];   // (g_ubar, g_vbar) on the current region
-    __shared__ float2 XS[XW * XH];  // state x_i on the tile grown by 1 (clamped)
-    __shared__ double red[NT / 64];
+    constexpr int TP = TW * TH;
+    static_assert(TP % NT == 0, "tile pixels must split evenly over the threads");
+    constexpr int PT = TP / NT;
+    constexpr int PH = (NPX - TP + NT - 1) / NT;
+    constexpr int P = PT + PH;
+    __shared__ float2 GB[2][NPX];  // (g_ubar, g_vbar) on the current region
+    __shared__ double red[2][NT / 64];
 
     const int b = blockIdx.y;
     const int64_t off = (int64_t)b * a.bstride;
@@ -118,149 +130,192 @@ __global__ __launch_bounds__(NT) void hs_adj_tb_kernel(AdjTbArgs a, Taps taps) {
     const int x1 = min(x0 + TW, W), y1 = min(y0 + TH, H);
     const int gx0 = max(x0 - m, 0), gy0 = max(y0 - m, 0);
     const int gx1 = min(x1 + m, W), gy1 = min(y1 + m, H);
-    const int RW = gx1 - gx0, RH = gy1 - gy0, npx = RW * RH;
+    const int RW = gx1 - gx0;
     const bool clipped = (x0 - m < 0) || (y0 - m < 0) || (x1 + m > W) || (y1 + m > H);
-    const float *Ix = a.Ix + off, *Iy = a.Iy + off, *It = a.It + off;
+    const int tw = x1 - x0, th = y1 - y0;
+    const int tx = x0 - gx0, ty = y0 - gy0;  // tile origin in region coordinates
 
+    // region coordinates (ry << 16 | rx) of every slot, -1 if none
     int pk[P];
-    float cx[P], cy[P], ct[P], cd[P];
-    float ax[P], ay[P], at[P];
 #pragma unroll
-    for (int j = 0; j < P; ++j) {
+    for (int j = 0; j < PT; ++j) {
         const int i = threadIdx.x + j * NT;
-        pk[j] = -1;
-        cx[j] = cy[j] = ct[j] = 0.0f;
-        cd[j] = 1.0f;
-        ax[j] = ay[j] = at[j] = 0.0f;
-        if (i < npx) {
-            const int ry = i / RW, rx = i - ry * RW;
-            pk[j] = (ry << 16) | rx;
-            const int64_t g = (int64_t)(gy0 + ry) * W + gx0 + rx;
-            const float ix = Ix[g], iy = Iy[g];
-            cx[j] = ix;
-            cy[j] = iy;
-            ct[j] = It[g];
-            cd[j] = (alpha2 + ix * ix) + iy * iy;
-            G[ry * PITCH + rx] = make_float2(a.gu_in[off + g], a.gv_in[off + g]);
-            // the accumulators continue from their stored value and take one
-            // step's term at a time, so the sums do not depend on how the
-            // reverse sweep is cut into launches
-            const int gx = gx0 + rx, gy = gy0 + ry;
-            if (gx >= x0 && gx < x1 && gy >= y0 && gy < y1) {
-                ax[j] = a.gIx[off + g];
-                ay[j] = a.gIy[off + g];
-                at[j] = a.gIt[off + g];
+        const int ly = i / TW, lx = i - ly * TW;
+        pk[j] = (lx < tw && ly < th) ? ((ty + ly) << 16) | (tx + lx) : -1;
+    }
+    {
+        const int lw = tx, rw = gx1 - x1, sw = lw + rw;
+        const int n_top = ty * RW, n_bot = (gy1 - y1) * RW, n_side = th * sw;
+#pragma unroll
+        for (int j = 0; j < PH; ++j) {
+            const int h = threadIdx.x + j * NT;
+            int rx = -1, ry = 0;
+            if (h < n_top) {
+                ry = h / RW;
+                rx = h - ry * RW;
+            } else if (h < n_top + n_bot) {
+                const int q = h - n_top;
+                ry = ty + th + q / RW;
+                rx = q % RW;
+            } else if (h < n_top + n_bot + n_side) {
+                const int q = h - n_top - n_bot;
+                const int r = q / sw, c = q - r * sw;
+                ry = ty + r;
+                rx = c < lw ? c : tx + tw + (c - lw);
             }
+            pk[PT + j] = rx < 0 ? -1 : (ry << 16) | rx;
         }
     }
-    double tile_a2 = 0.0;        // running per-tile sum, held by thread 0
+
+    const float *Ix = a.Ix + off, *Iy = a.Iy + off;
+    float cx[P], cy[P], cd[P];
+    float2 g[P];
+    float ax[PT], ay[PT], at[PT];
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        cx[j] = cy[j] = 0.0f;
+        cd[j] = 1.0f;
+        g[j] = make_float2(0.0f, 0.0f);
+        if (j < PT) ax[j] = ay[j] = at[j] = 0.0f;
+        if (pk[j] < 0) continue;
+        const int rx = pk[j] & 0xffff, ry = pk[j] >> 16;
+        const int64_t q = (int64_t)(gy0 + ry) * W + gx0 + rx;
+        const float ix = Ix[q], iy = Iy[q];
+        cx[j] = ix;
+        cy[j] = iy;
+        cd[j] = 1.0f / ((alpha2 + ix * ix) + iy * iy);  // reciprocal of den
+        g[j] = make_float2(a.gu_in[off + q], a.gv_in[off + q]);
+        // the accumulators continue from their stored value and take one
+        // step's term at a time, so the sums do not depend on how the
+        // reverse sweep is cut into launches
+        if (j < PT) {
+            ax[j] = a.gIx[off + q];
+            ay[j] = a.gIy[off + q];
+            at[j] = a.gIt[off + q];
+        }
+    }
+    double tile_a2 = 0.0;  // running per-tile sum, held by thread 0
     if (threadIdx.x == 0) tile_a2 = a.ga2[(int64_t)b * a.ntiles + t];
 
-    for (int step = 1; step <= m; ++step) {
-        // stage x_i (i = hi - step) on the tile grown by one, clamped
-        const float *XU = a.xu[step - 1] + off, *XV = a.xv[step - 1] + off;
-        for (int i = threadIdx.x; i < XW * XH; i += NT) {
-            const int ly = i / XW, lx = i - ly * XW;
-            const int64_t g = (int64_t)clampi(y0 - 1 + ly, 0, H - 1) * W +
-                              clampi(x0 - 1 + lx, 0, W - 1);
-            XS[i] = make_float2(XU[g], XV[g]);
-        }
-        __syncthreads();
-        // pass A on R_{step-1}: s, (g_ubar, g_vbar); accumulate on the tile
-        double a2 = 0.0;
-        {
-            const int e = m - step + 1;
-            const int rx0 = max(x0 - e, 0) - gx0, ry0 = max(y0 - e, 0) - gy0;
-            const int rx1 = min(x1 + e, W) - gx0, ry1 = min(y1 + e, H) - gy0;
+    // (t, ubar, vbar) of forward step hi - step at the tile slots
+    float lt[PT], lu[PT], lv[PT];
+    auto load_lin = [&](int step) {
+        const float *T0 = a.lt[step - 1] + off, *U0 = a.lub[step - 1] + off,
+                    *V0 = a.lvb[step - 1] + off;
 #pragma unroll
-            for (int j = 0; j < P; ++j) {
-                int pkj = pk[j];
-                asm volatile("" : "+v"(pkj));
-                const int rx = pkj & 0xffff, ry = pkj >> 16;
-                if (pkj < 0 || rx < rx0 || rx >= rx1 || ry < ry0 || ry >= ry1) continue;
-                const float2 g = G[ry * PITCH + rx];
-                const float s = -(cx[j] * g.x + cy[j] * g.y) / cd[j];
-                GB[ry * PITCH + rx] = make_float2(g.x + cx[j] * s, g.y + cy[j] * s);
-                const int gx = gx0 + rx, gy = gy0 + ry;
-                if (gx >= x0 && gx < x1 && gy >= y0 && gy < y1) {
-                    const float2 *xb = XS + (gy - y0 + 1) * XW + (gx - x0 + 1);
-                    float ub = 0.0f, vb = 0.0f;
-                    for (int d = 0; d < taps.nd; ++d) {
-                        const float2 q = xb[taps.oy[d] * XW + taps.ox[d]];
-                        ub = fmaf(taps.w[d], q.x, ub);
-                        vb = fmaf(taps.w[d], q.y, vb);
-                    }
-                    const float tt = ((cx[j] * ub + cy[j] * vb) + ct[j]) / cd[j];
-                    const float st2 = 2.0f * s * tt;
-                    ax[j] += (-g.x * tt + s * ub) - st2 * cx[j];
-                    ay[j] += (-g.y * tt + s * vb) - st2 * cy[j];
-                    at[j] += s;
-                    a2 += (double)(-s * tt);
-                }
+        for (int j = 0; j < PT; ++j) {
+            int pkj = pk[j];
+            asm volatile("" : "+v"(pkj));  // recompute the offsets per step
+            lt[j] = lu[j] = lv[j] = 0.0f;
+            if (pkj >= 0) {
+                const int64_t q = (int64_t)(gy0 + (pkj >> 16)) * W + gx0 + (pkj & 0xffff);
+                lt[j] = T0[q];
+                lu[j] = U0[q];
+                lv[j] = V0[q];
+            }
+        }
+    };
+
+    // pass A of step `step` on R_{m-step+1}: s, (g_ubar, g_vbar) into GB[buf],
+    // the tile's accumulator terms, and the step's part of the alpha sum
+    auto pass_a = [&](int step, int buf) {
+        const int e = m - step + 1;
+        const int rx0 = max(x0 - e, 0) - gx0, ry0 = max(y0 - e, 0) - gy0;
+        const int rx1 = min(x1 + e, W) - gx0, ry1 = min(y1 + e, H) - gy0;
+        double a2 = 0.0;
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            int pkj = pk[j];
+            asm volatile("" : "+v"(pkj));
+            const int rx = pkj & 0xffff, ry = pkj >> 16;
+            if (pkj < 0 || rx < rx0 || rx >= rx1 || ry < ry0 || ry >= ry1) continue;
+            const float s = -(cx[j] * g[j].x + cy[j] * g[j].y) * cd[j];
+            GB[buf][ry * PITCH + rx] = make_float2(g[j].x + cx[j] * s, g[j].y + cy[j] * s);
+            if (j < PT) {  // a tile pixel (always inside the region)
+                const float tt = lt[j], st2 = 2.0f * s * tt;
+                ax[j] += (-g[j].x * tt + s * lu[j]) - st2 * cx[j];
+                ay[j] += (-g[j].y * tt + s * lv[j]) - st2 * cy[j];
+                at[j] += s;
+                a2 += (double)(-s * tt);
             }
         }
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) a2 += __shfl_xor(a2, o, 64);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a2;
-        __syncthreads();
+        if ((threadIdx.x & 63) == 0) red[step & 1][threadIdx.x >> 6] = a2;
+    };
+    // thread 0 adds the step's per-wave sums, in wave order, after the barrier
+    auto add_a2 = [&](int step) {
         if (threadIdx.x == 0) {
             double r = 0.0;
-            for (int i = 0; i < NT / 64; ++i) r += red[i];
+            for (int i = 0; i < NT / 64; ++i) r += red[step & 1][i];
             tile_a2 += r;
         }
-        // pass B on R_step: g <- A^T g_b (gather over sources that clamp onto p)
-        {
-            const int e = m - step;
-            const int rx0 = max(x0 - e, 0) - gx0, ry0 = max(y0 - e, 0) - gy0;
-            const int rx1 = min(x1 + e, W) - gx0, ry1 = min(y1 + e, H) - gy0;
+    };
+
+    load_lin(1);
+    pass_a(1, 0);
+    __syncthreads();
+    add_a2(1);
+
+    for (int step = 1; step <= m; ++step) {
+        const int buf = (step - 1) & 1;
+        // pass B on R_{m-step}: g <- A^T g_b (gather over sources that clamp onto p)
+        const int e = m - step;
+        const int rx0 = max(x0 - e, 0) - gx0, ry0 = max(y0 - e, 0) - gy0;
+        const int rx1 = min(x1 + e, W) - gx0, ry1 = min(y1 + e, H) - gy0;
+        const float2 *GBb = GB[buf];
 #pragma unroll
-            for (int j = 0; j < P; ++j) {
-                int pkj = pk[j];
-                asm volatile("" : "+v"(pkj));
-                const int rx = pkj & 0xffff, ry = pkj >> 16;
-                if (pkj < 0 || rx < rx0 || rx >= rx1 || ry < ry0 || ry >= ry1) continue;
-                const int gx = gx0 + rx, gy = gy0 + ry;
-                float au = 0.0f, av = 0.0f;
-                if (!clipped || (gx > 0 && gx < W - 1 && gy > 0 && gy < H - 1)) {
-                    const float2 *gb = GB + ry * PITCH + rx;
-                    for (int d = 0; d < taps.nd; ++d) {
-                        const float2 q = gb[-(taps.oy[d] * PITCH + taps.ox[d])];
-                        au = fmaf(taps.w[d], q.x, au);
-                        av = fmaf(taps.w[d], q.y, av);
-                    }
-                } else {
-                    for (int d = 0; d < taps.nd; ++d) {
-                        int lox, hix, loy, hiy;
-                        tr_range(gx, taps.ox[d], W, lox, hix);
-                        tr_range(gy, taps.oy[d], H, loy, hiy);
-                        for (int qy = loy; qy <= hiy; ++qy)
-                            for (int qx = lox; qx <= hix; ++qx) {
-                                const float2 q = GB[(qy - gy0) * PITCH + (qx - gx0)];
-                                au = fmaf(taps.w[d], q.x, au);
-                                av = fmaf(taps.w[d], q.y, av);
-                            }
-                    }
+        for (int j = 0; j < P; ++j) {
+            int pkj = pk[j];
+            asm volatile("" : "+v"(pkj));
+            const int rx = pkj & 0xffff, ry = pkj >> 16;
+            if (pkj < 0 || rx < rx0 || rx >= rx1 || ry < ry0 || ry >= ry1) continue;
+            const int gx = gx0 + rx, gy = gy0 + ry;
+            float au = 0.0f, av = 0.0f;
+            if (!clipped || (gx > 0 && gx < W - 1 && gy > 0 && gy < H - 1)) {
+                const float2 *gb = GBb + ry * PITCH + rx;
+                for (int d = 0; d < taps.nd; ++d) {
+                    const float2 q = gb[-(taps.oy[d] * PITCH + taps.ox[d])];
+                    au = fmaf(taps.w[d], q.x, au);
+                    av = fmaf(taps.w[d], q.y, av);
                 }
-                G[ry * PITCH + rx] = make_float2(au, av);
+            } else {
+                for (int d = 0; d < taps.nd; ++d) {
+                    int lox, hix, loy, hiy;
+                    tr_range(gx, taps.ox[d], W, lox, hix);
+                    tr_range(gy, taps.oy[d], H, loy, hiy);
+                    for (int qy = loy; qy <= hiy; ++qy)
+                        for (int qx = lox; qx <= hix; ++qx) {
+                            const float2 q = GBb[(qy - gy0) * PITCH + (qx - gx0)];
+                            au = fmaf(taps.w[d], q.x, au);
+                            av = fmaf(taps.w[d], q.y, av);
+                        }
+                }
             }
+            g[j] = make_float2(au, av);
         }
-        __syncthreads();
+        if (step < m) {
+            // (loaded here rather than during pass B: holding them across the
+            // gather would spill registers)
+            load_lin(step + 1);
+            // GB[buf ^ 1] was last read in pass B of step - 1, before the
+            // previous barrier
+            pass_a(step + 1, buf ^ 1);
+            __syncthreads();
+            add_a2(step + 1);
+        }
     }
 
 #pragma unroll
-    for (int j = 0; j < P; ++j) {
+    for (int j = 0; j < PT; ++j) {
         if (pk[j] < 0) continue;
         const int rx = pk[j] & 0xffff, ry = pk[j] >> 16;
-        const int gx = gx0 + rx, gy = gy0 + ry;
-        if (gx >= x0 && gx < x1 && gy >= y0 && gy < y1) {
-            const int64_t g = off + (int64_t)gy * W + gx;
-            const float2 q = G[ry * PITCH + rx];
-            a.gu_out[g] = q.x;
-            a.gv_out[g] = q.y;
-            a.gIx[g] = ax[j];
-            a.gIy[g] = ay[j];
-            a.gIt[g] = at[j];
-        }
+        const int64_t q = off + (int64_t)(gy0 + ry) * W + gx0 + rx;
+        a.gu_out[q] = g[j].x;
+        a.gv_out[q] = g[j].y;
+        a.gIx[q] = ax[j];
+        a.gIy[q] = ay[j];
+        a.gIt[q] = at[j];
     }
     if (threadIdx.x == 0) a.ga2[(int64_t)b * a.ntiles + t] = tile_a2;
 }
@@ -356,8 +411,9 @@ extern "C" int hmx_hs_ckpt_count(int iters, int ckpt_every) {
 
 extern "C" int64_t hmx_hs_adjoint_workspace(int B, int H, int W, int ckpt_every) {
     const int64_t n = (int64_t)B * H * W;
-    // Ix Iy It | gIx gIy gIt | 2 gradient ping-pong pairs | segment states
-    const int64_t floats = 3 * n + 3 * n + 4 * n + 2 * n * (ckpt_every > 1 ? ckpt_every - 1 : 0);
+    // Ix Iy It | gIx gIy gIt | 2 gradient ping-pong pairs | 2 recompute state
+    // ping-pong pairs | (t, ubar, vbar) of every step of one segment
+    const int64_t floats = 3 * n + 3 * n + 4 * n + 4 * n + 3 * n * std::max(ckpt_every, 1);
     const int64_t doubles = (int64_t)B * adj_tiles(H, W);
     return floats * 4 + doubles * 8 + (int64_t)B * 4 + 256;
 }
@@ -433,9 +489,9 @@ extern "C" int hmx_hs_adjoint(const float *I1, const float *I2, int B, int H, in
     float *Ix = ws, *Iy = ws + n, *It = ws + 2 * n;
     float *gIx = ws + 3 * n, *gIy = ws + 4 * n, *gIt = ws + 5 * n;
     float *g[2][2] = {{ws + 6 * n, ws + 7 * n}, {ws + 8 * n, ws + 9 * n}};
-    float *seg = ws + 10 * n;
-    const int nseg = ckpt_every > 1 ? ckpt_every - 1 : 0;
-    double *ga2 = (double *)(((uintptr_t)(seg + 2 * n * nseg) + 255) & ~(uintptr_t)255);
+    float *rs[2][2] = {{ws + 10 * n, ws + 11 * n}, {ws + 12 * n, ws + 13 * n}};
+    float *lin = ws + 14 * n;  // slot j: t, ubar, vbar of the segment's step j
+    double *ga2 = (double *)(((uintptr_t)(lin + 3 * n * ckpt_every) + 255) & ~(uintptr_t)255);
     float *alpha2 = (float *)(ga2 + (int64_t)B * ntiles);
 
     hipLaunchKernelGGL(alpha_sq_kernel, dim3((B + 63) / 64), dim3(64), 0, s, alpha, alpha2, B);
@@ -467,22 +523,20 @@ extern "C" int hmx_hs_adjoint(const float *I1, const float *I2, int B, int H, in
     const int nck = hmx_hs_ckpt_count(iters, ckpt_every);
     for (int c = nck - 1; c >= 0; --c) {
         const int len = (iters - c * ckpt_every) < ckpt_every ? iters - c * ckpt_every : ckpt_every;
-        auto state_u = [&](int j) -> float * {
-            return j == 0 ? (float *)ckpt + (int64_t)(2 * c) * n : seg + (int64_t)(2 * (j - 1)) * n;
-        };
-        auto state_v = [&](int j) -> float * {
-            return j == 0 ? (float *)ckpt + (int64_t)(2 * c + 1) * n
-                          : seg + (int64_t)(2 * (j - 1) + 1) * n;
-        };
-        // recompute states 1..len-1 of the segment, k steps per launch, every
-        // intermediate state written out (bit-identical to the forward pass)
-        for (int j0 = 0; j0 < len - 1;) {
-            const int kk = std::min(ADJ_RECOMPUTE_K, len - 1 - j0);
+        // recompute the segment's len steps from its checkpoint, k steps per
+        // launch, keeping each step's (t, ubar, vbar); the states themselves
+        // are bit-identical to the forward pass and only ping-pong here
+        const float *cu = ckpt + (int64_t)(2 * c) * n, *cv = ckpt + (int64_t)(2 * c + 1) * n;
+        int rcur = 0;
+        for (int j0 = 0; j0 < len;) {
+            const int kk = std::min(ADJ_RECOMPUTE_K, len - j0);
+            float *lt = lin + (int64_t)3 * j0 * n;
             rc = launch_hs_tb(ADJ_RECOMPUTE_VARIANT, Ix, Iy, It, nullptr, 0.0f, H, W, T.av, kk,
-                              state_u(j0), state_v(j0), state_u(j0 + kk), state_v(j0 + kk), s, B,
-                              alpha2, kk > 1 ? state_u(j0 + 1) : nullptr,
-                              kk > 1 ? state_v(j0 + 1) : nullptr, 2 * n);
+                              j0 == 0 ? cu : rs[rcur][0], j0 == 0 ? cv : rs[rcur][1],
+                              rs[rcur ^ 1][0], rs[rcur ^ 1][1], s, B, alpha2, lt, lt + n,
+                              lt + 2 * n, 3 * n);
             if (rc) return rc;
+            rcur ^= 1;
             j0 += kk;
         }
         // reverse steps len-1 .. 0, up to ADJ_MMAX per launch
@@ -490,8 +544,10 @@ extern "C" int hmx_hs_adjoint(const float *I1, const float *I2, int B, int H, in
             const int mm = std::min(ADJ_MMAX, hi);
             a.m = mm;
             for (int st = 1; st <= mm; ++st) {
-                a.xu[st - 1] = state_u(hi - st);
-                a.xv[st - 1] = state_v(hi - st);
+                const float *lt = lin + (int64_t)3 * (hi - st) * n;
+                a.lt[st - 1] = lt;
+                a.lub[st - 1] = lt + n;
+                a.lvb[st - 1] = lt + 2 * n;
             }
             a.gu_in = g[cur][0];
             a.gv_in = g[cur][1];

@@ -39,13 +39,16 @@ struct HsArgs {
     float alpha2;
     int H, W, k, tiles_x, ntiles;
     int64_t bstride;        // elements between consecutive pairs (blockIdx.y)
-    float *su, *sv;         // optional intermediate-state output (see HsImage)
-    int64_t sstride;
+    float *lt, *lub, *lvb;  // optional per-step linearisation output (see HsImage)
+    int64_t lstride;
 };
 
-template <int NT, int TW, int TH, int KMAX, int NTAP, bool SOUT = false>
-__global__ __launch_bounds__(NT) void hs_tb_kernel(HsArgs a, Taps taps) {
+// The linearisation-output form (the adjoint's segment recompute) is compiled
+// for 8 waves per SIMD so its extra stores do not cost a work-group per CU.
+template <int NT, int TW, int TH, int KMAX, int NTAP, bool LOUT = false>
+__global__ __launch_bounds__(NT, LOUT ? 8 : 1) void hs_tb_kernel(HsArgs a, Taps taps) {
     __shared__ float2 uv[(TW + 2 * KMAX) * (TH + 2 * KMAX)];
+    __shared__ float lin[LOUT ? 3 * TW * TH : 1];
     const int64_t off = (int64_t)blockIdx.y * a.bstride;  // pair of a batch
     HsImage im;
     im.Ix = a.Ix + off;
@@ -60,13 +63,15 @@ __global__ __launch_bounds__(NT) void hs_tb_kernel(HsArgs a, Taps taps) {
     im.H = a.H;
     im.W = a.W;
     im.tiles_x = a.tiles_x;
-    if (SOUT) {
-        im.su = a.su + off;
-        im.sv = a.sv + off;
-        im.sstride = a.sstride;
+    if (LOUT) {
+        im.lt = a.lt + off;
+        im.lub = a.lub + off;
+        im.lvb = a.lvb + off;
+        im.lstride = a.lstride;
     }
     const HsTapTable<TW + 2 * KMAX> tp(taps);
-    hs_tile<NT, TW, TH, KMAX, NTAP, SOUT>(im, xcd_remap(blockIdx.x, a.ntiles), a.k, tp, uv);
+    hs_tile<NT, TW, TH, KMAX, NTAP, LOUT>(im, xcd_remap(blockIdx.x, a.ntiles), a.k, tp, uv,
+                                          LOUT ? lin : nullptr);
 }
 
 template <int NT, int TW, int TH, int KMAX, int S, int NTAP>
@@ -157,13 +162,15 @@ int launch_hs_tb(int variant, const float *Ix, const float *Iy, const float *It,
                  const float *den, float alpha2, int H, int W, const Taps &taps, int k,
                  const float *u_in, const float *v_in, float *u_out, float *v_out,
                  hipStream_t s, int B = 1, const float *alpha2_b = nullptr,
-                 float *su = nullptr, float *sv = nullptr, int64_t sstride = 0) {
+                 float *lt = nullptr, float *lub = nullptr, float *lvb = nullptr,
+                 int64_t lstride = 0) {
     const HsVariant &V = kVariants[variant];
     HsArgs a;
     a.alpha2_b = alpha2_b;
-    a.su = su;
-    a.sv = sv;
-    a.sstride = sstride;
+    a.lt = lt;
+    a.lub = lub;
+    a.lvb = lvb;
+    a.lstride = lstride;
     a.bstride = (int64_t)H * W;
     a.Ix = Ix;
     a.Iy = Iy;
@@ -181,15 +188,16 @@ int launch_hs_tb(int variant, const float *Ix, const float *Iy, const float *It,
     const int tiles_y = (H + V.th - 1) / V.th;
     a.ntiles = a.tiles_x * tiles_y;
     auto kern = V.kernel[tap_class(taps.nd)];
-    if (su) {
-        // state output exists only for the 1024-thread 64x32 k<=8 shape
-        static void (*const kStateKernels[5])(HsArgs, Taps) = {
+    if (lt) {
+        // linearisation output exists only for the 1024-thread 64x32 k<=8 shape
+        static void (*const kLinKernels[5])(HsArgs, Taps) = {
             hs_tb_kernel<1024, 64, 32, 8, 4, true>, hs_tb_kernel<1024, 64, 32, 8, 5, true>,
             hs_tb_kernel<1024, 64, 32, 8, 8, true>, hs_tb_kernel<1024, 64, 32, 8, 9, true>,
             hs_tb_kernel<1024, 64, 32, 8, 0, true>};
         if (V.strip || V.nt != 1024 || V.tw != 64 || V.th != 32 || V.kmax != 8)
-            return fail(HMX_ERR_UNSUPPORTED, "hs_tb: state output needs the 1024x64x32x8 variant");
-        kern = kStateKernels[tap_class(taps.nd)];
+            return fail(HMX_ERR_UNSUPPORTED,
+                        "hs_tb: linearisation output needs the 1024x64x32x8 variant");
+        kern = kLinKernels[tap_class(taps.nd)];
     }
     hipLaunchKernelGGL(kern, dim3(a.ntiles, B), dim3(V.nt), 0, s, a, taps);
     HMX_LAUNCH_CHECK("hs_tb");

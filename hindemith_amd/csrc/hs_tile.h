@@ -75,10 +75,14 @@ __device__ __forceinline__ float hs_update(float ix, float iy, float it, float d
 // a.lub, a.lvb, staged through lin (3 x TW x TH floats of LDS) and stored in
 // tile order between the step's two barriers (a compile-time switch: the
 // extra work and LDS stay out of the kernels that do not need it).
-template <int NT, int TW, int TH, int KMAX, int NTAP, bool LOUT = false>
+// DB: uv holds two windows; step j reads window (j-1)&1 and writes its result
+// straight into the other, so a step needs one barrier and no new values are
+// held in registers across it.
+template <int NT, int TW, int TH, int KMAX, int NTAP, bool LOUT = false, bool DB = false>
 __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
                                         const HsTapTable<TW + 2 * KMAX> &tp, float2 *uv,
                                         float *lin = nullptr) {
+    static_assert(!(LOUT && DB), "the linearisation output needs the two-barrier step");
     constexpr int PITCH = TW + 2 * KMAX, RHM = TH + 2 * KMAX;
     constexpr int NPX = PITCH * RHM;
     constexpr int P = (NPX + NT - 1) / NT;
@@ -124,6 +128,8 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
         const int e = k - it;
         const int rx0 = max(x0 - e, 0) - gx0, ry0 = max(y0 - e, 0) - gy0;
         const int rx1 = min(x1 + e, a.W) - gx0, ry1 = min(y1 + e, a.H) - gy0;
+        const float2 *uvr = DB ? uv + ((it - 1) & 1) * NPX : uv;
+        float2 *uvw = DB ? uv + (it & 1) * NPX : uv;
         float nu[P], nv[P];
         bool act[P];
 #pragma unroll
@@ -138,7 +144,7 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
             if (act[j]) {
                 float ub = 0.0f, vb = 0.0f;
                 if (!clipped) {
-                    const float2 *base = uv + ry * PITCH + rx;
+                    const float2 *base = uvr + ry * PITCH + rx;
 #pragma unroll
                     for (int d = 0; d < (NTAP ? NTAP : HS_MAX_TAPS); ++d) {
                         if (NTAP || d < tp.nd) {
@@ -154,13 +160,14 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
                         if (NTAP || d < tp.nd) {
                             const int nx = clampi(gx + tp.ox[d], 0, a.W - 1) - gx0;
                             const int ny = clampi(gy + tp.oy[d], 0, a.H - 1) - gy0;
-                            const float2 q = uv[ny * PITCH + nx];
+                            const float2 q = uvr[ny * PITCH + nx];
                             ub = fmaf(tp.w[d], q.x, ub);
                             vb = fmaf(tp.w[d], q.y, vb);
                         }
                     }
                 }
                 const float tt = hs_update(cx[j], cy[j], ct[j], cd[j], ub, vb, nu[j], nv[j]);
+                if (DB) uvw[ry * PITCH + rx] = make_float2(nu[j], nv[j]);
                 if (LOUT) {
                     const int lx = gx0 + rx - x0, ly = gy0 + ry - y0;
                     if (lx >= 0 && lx < TW && ly >= 0 && ly < TH) {
@@ -171,6 +178,12 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
                     }
                 }
             }
+        }
+        if (DB) {
+            // the window written here was last read in step it-1, before the
+            // previous barrier
+            __syncthreads();
+            continue;
         }
         __syncthreads();
 #pragma unroll
@@ -200,7 +213,7 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
         const int gx = gx0 + rx, gy = gy0 + ry;
         if (gx >= x0 && gx < x1 && gy >= y0 && gy < y1) {
             const int64_t g = (int64_t)gy * a.W + gx;
-            const float2 q = uv[ry * PITCH + rx];
+            const float2 q = (DB ? uv + (k & 1) * NPX : uv)[ry * PITCH + rx];
             a.u_out[g] = q.x;
             a.v_out[g] = q.y;
         }

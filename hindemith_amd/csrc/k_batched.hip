@@ -102,10 +102,11 @@ __device__ inline float setup_tap(const float *__restrict__ x, int H, int W, int
 
 // WPE: waves per SIMD the kernel is compiled to allow (the compiler must fit
 // its VGPRs and SGPRs to it), from which the resident grid is derived.
-template <int NT, int TW, int TH, int KMAX, int NTAP, int WPE>
+// DB: two LDS windows, one barrier per Jacobi step (see hs_tile).
+template <int NT, int TW, int TH, int KMAX, int NTAP, int WPE, bool DB = false>
 __global__ __launch_bounds__(NT, WPE) void hs_batched_kernel(BatchArgs b, SetupTaps st,
                                                              Taps taps) {
-    __shared__ float2 uv[(TW + 2 * KMAX) * (TH + 2 * KMAX)];
+    __shared__ float2 uv[(DB ? 2 : 1) * (TW + 2 * KMAX) * (TH + 2 * KMAX)];
     __shared__ int s_claim;
     const HsTapTable<TW + 2 * KMAX> tp(taps);
     unsigned arrivals = 0;
@@ -168,7 +169,7 @@ __global__ __launch_bounds__(NT, WPE) void hs_batched_kernel(BatchArgs b, SetupT
             im.H = P.H;
             im.W = P.W;
             im.tiles_x = P.tiles_x;
-            hs_tile<NT, TW, TH, KMAX, NTAP>(im, w - P.tile_base, kk, tp, uv);
+            hs_tile<NT, TW, TH, KMAX, NTAP, false, DB>(im, w - P.tile_base, kk, tp, uv);
         }
         arrivals += gridDim.x;
         if (!grid_sync(b.barrier, arrivals, b.status, b.spin_limit)) return;
@@ -201,14 +202,21 @@ struct BVariant {
 #define HB_V(NT, TW, TH, K, WPE)                                                          \
     {NT, TW, TH, K, WPE,                                                                  \
      {hs_batched_kernel<NT, TW, TH, K, 8, WPE>, hs_batched_kernel<NT, TW, TH, K, 0, WPE>}}
+#define HB_D(NT, TW, TH, K, WPE)                                                          \
+    {NT, TW, TH, K, WPE,                                                                  \
+     {hs_batched_kernel<NT, TW, TH, K, 8, WPE, true>,                                     \
+      hs_batched_kernel<NT, TW, TH, K, 0, WPE, true>}}
 static const BVariant kBVariants[] = {
     HB_V(1024, 64, 32, 4, 8),   // 0: two work-groups per CU
     HB_V(1024, 64, 32, 8, 8),   // 1: two work-groups per CU
     HB_V(1024, 128, 32, 8, 4),  // 2: one
     HB_V(512, 64, 32, 4, 4),    // 3: two
     HB_V(1024, 64, 64, 8, 4),   // 4: one
+    HB_D(1024, 64, 32, 4, 8),   // 5: double-buffered window, two per CU
+    HB_D(1024, 64, 32, 8, 8),   // 6: double-buffered window, two per CU
 };
 #undef HB_V
+#undef HB_D
 static constexpr int kNumBVariants = sizeof(kBVariants) / sizeof(kBVariants[0]);
 
 // A cached launch: device descriptors, barrier words and (optionally) the

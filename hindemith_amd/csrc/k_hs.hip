@@ -45,9 +45,10 @@ struct HsArgs {
 
 // The linearisation-output form (the adjoint's segment recompute) is compiled
 // for 8 waves per SIMD so its extra stores do not cost a work-group per CU.
-template <int NT, int TW, int TH, int KMAX, int NTAP, bool LOUT = false>
+// DB: two LDS windows, one barrier per step (see hs_tile).
+template <int NT, int TW, int TH, int KMAX, int NTAP, bool LOUT = false, bool DB = false>
 __global__ __launch_bounds__(NT, LOUT ? 8 : 1) void hs_tb_kernel(HsArgs a, Taps taps) {
-    __shared__ float2 uv[(TW + 2 * KMAX) * (TH + 2 * KMAX)];
+    __shared__ float2 uv[(DB ? 2 : 1) * (TW + 2 * KMAX) * (TH + 2 * KMAX)];
     __shared__ float lin[LOUT ? 3 * TW * TH : 1];
     const int64_t off = (int64_t)blockIdx.y * a.bstride;  // pair of a batch
     HsImage im;
@@ -70,8 +71,8 @@ __global__ __launch_bounds__(NT, LOUT ? 8 : 1) void hs_tb_kernel(HsArgs a, Taps 
         im.lstride = a.lstride;
     }
     const HsTapTable<TW + 2 * KMAX> tp(taps);
-    hs_tile<NT, TW, TH, KMAX, NTAP, LOUT>(im, xcd_remap(blockIdx.x, a.ntiles), a.k, tp, uv,
-                                          LOUT ? lin : nullptr);
+    hs_tile<NT, TW, TH, KMAX, NTAP, LOUT, DB>(im, xcd_remap(blockIdx.x, a.ntiles), a.k, tp, uv,
+                                              LOUT ? lin : nullptr);
 }
 
 template <int NT, int TW, int TH, int KMAX, int S, int NTAP>
@@ -120,6 +121,12 @@ static int tap_class(int nd) {
      {hs_tb_kernel<NT, TW, TH, K, 4>, hs_tb_kernel<NT, TW, TH, K, 5>,                  \
       hs_tb_kernel<NT, TW, TH, K, 8>, hs_tb_kernel<NT, TW, TH, K, 9>,                  \
       hs_tb_kernel<NT, TW, TH, K, 0>}}
+// double-buffered window, one barrier per step
+#define HS_D(NT, TW, TH, K)                                                             \
+    {NT, TW, TH, K, 0,                                                                  \
+     {hs_tb_kernel<NT, TW, TH, K, 4, false, true>, hs_tb_kernel<NT, TW, TH, K, 5, false, true>, \
+      hs_tb_kernel<NT, TW, TH, K, 8, false, true>, hs_tb_kernel<NT, TW, TH, K, 9, false, true>, \
+      hs_tb_kernel<NT, TW, TH, K, 0, false, true>}}
 // column-strip variants (S pixels per thread per column)
 #define HS_S(NT, TW, TH, K, S)                                                          \
     {NT, TW, TH, K, S,                                                                  \
@@ -143,8 +150,15 @@ static const HsVariant kVariants[] = {
     HS_S(576, 64, 24, 4, 4),  // 13: region 72x32
     HS_S(1024, 56, 56, 4, 4), // 14: region 64x64
     HS_S(1024, 120, 24, 4, 4),// 15: region 128x32
+    HS_D(1024, 64, 32, 4),    // 16: double-buffered window
+    HS_D(1024, 64, 32, 8),    // 17
+    HS_D(1024, 128, 32, 4),   // 18
+    HS_D(1024, 64, 64, 4),    // 19
+    HS_D(512, 64, 32, 4),     // 20
+    HS_D(1024, 128, 32, 8),   // 21
 };
 #undef HS_V
+#undef HS_D
 #undef HS_S
 static constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 

@@ -48,7 +48,7 @@ def parse():
                    help="skip the secondary workloads (4K, bandwidth kernels, batched, adjoint)")
     p.add_argument("--copy-mb", type=int, default=0,
                    help="only run a plain device copy of this many MiB (counter calibration)")
-    p.add_argument("--secondary", choices=["4k", "bandwidth", "batched", "adjoint"],
+    p.add_argument("--secondary", choices=["4k", "bandwidth", "batched", "adjoint", "frontend"],
                    default=None, help="only run this secondary workload and print its JSON")
     p.add_argument("--traffic-sweep", action="store_true",
                    help="only launch every (variant, k) of the fused Jacobi kernel 3 times, in "
@@ -252,6 +252,52 @@ def extra_batched(S):
             "variant": vi, "k": k}
 
 
+def extra_frontend(S):
+    """The drop-in @hm path: the canonical Horn-Schunck program at 1920x1080,
+    500 iterations, called with host numpy arrays, so the PCIe copies of the
+    frames and of the returned flow are inside the time (wall clock per call,
+    median of 3 after a warm-up call that compiles the record)."""
+    os.environ.setdefault("HM_GPU0", "1")
+    from hindemith_amd import hindemith as Hm
+    Hm.defines.update(nd20=1, nd21=1, nt20=8, nt21=8, ne20=2, ne21=2)
+    hm, Stencil = Hm.hm, Hm.Stencil
+
+    @hm
+    def hs_1080p(I1, I2, Dx, Dy, Dt, Avg, alpha2, u, v):
+        Ix = Dx * I1 + Dx * I2
+        Iy = Dy * I1 + Dy * I2
+        It = Dt * I2 - Dt * I1
+        den = alpha2 + Ix * Ix + Iy * Iy
+        for i in range(500):
+            ubar = Avg * u
+            vbar = Avg * v
+            t = (Ix * ubar + Iy * vbar + It) / den
+            u = ubar - Ix * t
+            v = vbar - Iy * t
+        return u, v
+
+    taps = S.hs_stencil_taps()
+    H, W = 1080, 1920
+    I1, I2, _, _ = S.hs_pair(H, W, seed=0, flow="affine")
+    st = {k: Stencil(*taps[k]) for k in ("Dx", "Dy", "Dt", "Avg")}
+
+    def call():
+        z = np.zeros_like(I1)
+        return hs_1080p(I1=I1, I2=I2, alpha2=np.float32(1.0), u=z, v=z.copy(), **st)
+
+    call()
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        call()
+        times.append(time.perf_counter() - t0)
+    sec = float(np.median(times))
+    return {"workload": f"@hm canonical HS program, {W}x{H} fp32 500 it, host arrays",
+            "value": round(H * W * 500 / sec / 1e6, 1), "unit": "Mpixel*iter/s",
+            "ms_per_call": round(sec * 1e3, 3),
+            "plan": [l for l in hs_1080p.plan().splitlines() if l.strip()][:4]}
+
+
 def extra_adjoint(S):
     """8 pairs at 512x512, 200 iterations, checkpoints every 20: time of the
     hand-written backward vs the forward."""
@@ -331,7 +377,8 @@ def main():
     taps = S.hs_stencil_taps()
     if args.secondary:
         fn = {"4k": lambda: extra_4k(K, S, taps), "bandwidth": lambda: extra_bandwidth(K, S),
-              "batched": lambda: extra_batched(S), "adjoint": lambda: extra_adjoint(S)}
+              "batched": lambda: extra_batched(S), "adjoint": lambda: extra_adjoint(S),
+              "frontend": lambda: extra_frontend(S)}
         print(json.dumps({args.secondary: fn[args.secondary]()}))
         return 0
     I1h, I2h, fx, fy = S.hs_pair(H, W, seed=0, flow="affine")
@@ -455,7 +502,8 @@ def main():
         for name, fn in (("hs_3840x2160", lambda: extra_4k(K, S, taps)),
                          ("bandwidth_2048x2048", lambda: extra_bandwidth(K, S)),
                          ("batched_vs_plain", lambda: extra_batched(S)),
-                         ("adjoint_vs_forward", lambda: extra_adjoint(S))):
+                         ("adjoint_vs_forward", lambda: extra_adjoint(S)),
+                         ("hm_frontend_host_arrays", lambda: extra_frontend(S))):
             try:
                 sec[name] = fn()
             except Exception as e:  # a secondary workload must not hide the headline

@@ -48,7 +48,8 @@ def parse():
                    help="skip the secondary workloads (4K, bandwidth kernels, batched, adjoint)")
     p.add_argument("--copy-mb", type=int, default=0,
                    help="only run a plain device copy of this many MiB (counter calibration)")
-    p.add_argument("--secondary", choices=["4k", "bandwidth", "batched", "adjoint", "frontend"],
+    p.add_argument("--secondary",
+                   choices=["4k", "bandwidth", "batched", "adjoint", "frontend", "hscg"],
                    default=None, help="only run this secondary workload and print its JSON")
     p.add_argument("--traffic-sweep", action="store_true",
                    help="only launch every (variant, k) of the fused Jacobi kernel 3 times, in "
@@ -110,6 +111,15 @@ def cpu_baseline(I1, I2, taps, H, W):
             "kind": "port",
             "sample": f"oracle C/OpenMP restatement, {W}x{H} setup + {it} Jacobi iterations, "
                       f"median of 5 runs after 1 warm-up ({med:.2f} s each)"}
+
+
+def endpoint_error(u, v, fx, fy, margin=8):
+    """Mean endpoint error of the flow against the true synthetic flow, over the
+    interior (margin px from each border)."""
+    sl = (slice(margin, -margin), slice(margin, -margin))
+    du = np.asarray(u, np.float64)[sl] - np.asarray(fx, np.float64)[sl]
+    dv = np.asarray(v, np.float64)[sl] - np.asarray(fy, np.float64)[sl]
+    return float(np.mean(np.sqrt(du * du + dv * dv)))
 
 
 def _time_ms(fn, reps, warm=2):
@@ -252,6 +262,37 @@ def extra_batched(S):
             "variant": vi, "k": k}
 
 
+def extra_hscg(K, S):
+    """The preconditioned-CG operations at 1920x1080 fp32, timed as hipGraph
+    replays: the 2x2 block-Jacobi preconditioner (28 B/px: read diag0, diag1,
+    offdiag, x0, x1; write y0, y1) and 8 HS operator powers (algorithmic bytes
+    as the reference's one launch per power: V[0] copy 16 B/px + 24 B/px per
+    power = 184 B/px; the blocked kernel's minimum traffic is 88 B/px: read Ix,
+    Iy, r once, write the 8 planes)."""
+    import torch
+    H, W, steps = 1080, 1920, 8
+    n = H * W
+    taps = S.hs_stencil_taps()
+    I1, I2, _, _ = S.hs_pair(H, W, seed=0, flow="affine")
+    Ix, Iy, It = K.hs_setup(torch.from_numpy(I1).cuda(), torch.from_numpy(I2).cuda(),
+                            taps["Dx"], taps["Dy"], taps["Dt"])
+    r0, r1 = -(Ix * It), -(Iy * It)
+    d0, d1, od = Ix * Ix, Iy * Iy, Ix * Iy
+    y = (torch.empty_like(r0), torch.empty_like(r0))
+    lap = S.laplacian5_taps()
+    ms_p = _time_graph_ms(lambda: K.hs_precondition(lap, d0, d1, od, r0, r1, out=y))
+    V = (torch.empty((steps, H, W), device="cuda"), torch.empty((steps, H, W), device="cuda"))
+    ms_m = _time_graph_ms(lambda: K.hs_matrix_powers(1.0, Ix, Iy, r0, r1, steps, out=V),
+                          launches=10)
+    alg = (16 + 24 * (steps - 1)) * n
+    return {"workload": f"{W}x{H} fp32",
+            "precondition": {"us": round(ms_p * 1e3, 2), "GBps": round(28 * n / ms_p / 1e6, 1)},
+            "matrix_powers_8": {"us": round(ms_m * 1e3, 2),
+                                "Mpx_power_per_s": round(n * (steps - 1) / ms_m / 1e3, 1),
+                                "GBps_algorithmic": round(alg / ms_m / 1e6, 1),
+                                "GBps_min_traffic": round(88 * n / ms_m / 1e6, 1)}}
+
+
 def extra_frontend(S):
     """The drop-in @hm path: the canonical Horn-Schunck program at 1920x1080,
     500 iterations, called with host numpy arrays, so the PCIe copies of the
@@ -378,7 +419,7 @@ def main():
     if args.secondary:
         fn = {"4k": lambda: extra_4k(K, S, taps), "bandwidth": lambda: extra_bandwidth(K, S),
               "batched": lambda: extra_batched(S), "adjoint": lambda: extra_adjoint(S),
-              "frontend": lambda: extra_frontend(S)}
+              "frontend": lambda: extra_frontend(S), "hscg": lambda: extra_hscg(K, S)}
         print(json.dumps({args.secondary: fn[args.secondary]()}))
         return 0
     I1h, I2h, fx, fy = S.hs_pair(H, W, seed=0, flow="affine")
@@ -477,8 +518,7 @@ def main():
 
     u_h = u.cpu().numpy()
     v_h = v.cpu().numpy()
-    import oracle
-    epe = oracle.endpoint_error(u_h, v_h, fx, fy)
+    epe = endpoint_error(u_h, v_h, fx, fy)
 
     out = {
         "metric": "Mpixel*iter/s (HS, 1 GPU)", "value": round(value, 1),
@@ -503,7 +543,8 @@ def main():
                          ("bandwidth_2048x2048", lambda: extra_bandwidth(K, S)),
                          ("batched_vs_plain", lambda: extra_batched(S)),
                          ("adjoint_vs_forward", lambda: extra_adjoint(S)),
-                         ("hm_frontend_host_arrays", lambda: extra_frontend(S))):
+                         ("hm_frontend_host_arrays", lambda: extra_frontend(S)),
+                         ("hs_cg_operations", lambda: extra_hscg(K, S))):
             try:
                 sec[name] = fn()
             except Exception as e:  # a secondary workload must not hide the headline

@@ -138,10 +138,13 @@ def run(record_id, kwargs):
                   "result_read")
             outs.append(int(val[0]) if dtype == np.int64 else float(val[0]))
             continue
-        shp = (int(shape[0]), int(shape[1])) if nd.value == 2 else (int(shape[0]),)
+        if nd.value == 3:  # Array3D (HSMatrixPowers result): C order
+            shp, order = (int(shape[0]), int(shape[1]), int(shape[2])), "C"
+        else:
+            shp = (int(shape[0]), int(shape[1])) if nd.value == 2 else (int(shape[0]),)
+            order = "F" if shape[2] else "C"
         src = kwargs.get(name) if syms[name][1].get("source") == "Header" else None
-        out = src if isinstance(src, np.ndarray) else np.empty(
-            shp, dtype=dtype, order="F" if shape[2] else "C")
+        out = src if isinstance(src, np.ndarray) else np.empty(shp, dtype=dtype, order=order)
         check(lib().hmx_result_read(rid, i, out.ctypes.data_as(ctypes.c_void_p), out.nbytes),
               "result_read")
         outs.append(out)

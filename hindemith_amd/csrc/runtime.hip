@@ -41,6 +41,12 @@ int hs_jacobi_run(const float *Ix, const float *Iy, const float *It, const float
                   float alpha2, int H, int W, const Taps &taps, int iters, float *u, float *v,
                   float *wu, float *wv, int variant, int k, hipStream_t s);
 void hs_choice(int H, int W, int &variant, int &k);
+int launch_hs_precond(const float *diag0, const float *diag1, const float *offdiag,
+                      const float *x0, const float *x1, float *y0, float *y1, int H, int W,
+                      const float *w_dev, float w_val, hipStream_t s);
+int launch_hs_powers(const float *Ix, const float *Iy, const float *r0, const float *r1,
+                     float *V0, float *V1, int H, int W, int steps, const void *lam_dev,
+                     int lam_dtype, float lam_val, hipStream_t s);
 
 namespace rt {
 
@@ -212,7 +218,7 @@ struct Sym {
     int kind = 0;
     int dtype = HMX_F32;
     bool header = false;
-    int64_t dim0 = 0, dim1 = 0, ndiags = 0, nnz = 0;
+    int64_t dim0 = 0, dim1 = 0, dim2 = 0, ndiags = 0, nnz = 0;
     bool col_mjr = false;
     // element grid the kernels see (memory order, Array2D.cpp:297-310)
     int H = 0, W = 0;
@@ -266,11 +272,12 @@ struct Group {
 };
 
 struct Node {
-    enum T { GROUP, SCALAR, CSR, LOOP, HS, RETURN } t;
+    enum T { GROUP, SCALAR, CSR, LOOP, HS, RETURN, PRECOND, POWERS, COPY3D } t;
     int idx = 0;                 // GROUP: groups[], SCALAR: sprogs[]
     int ub = 0;                  // LOOP / HS
     std::vector<Node> body;      // LOOP (and HS fallback)
-    std::vector<int> syms;       // CSR: y, A, x ; HS: u, v, Ix, Iy, It, den, Avg ; RETURN
+    std::vector<int> syms;       // CSR: y, A, x ; HS: u, v, Ix, Iy, It, den, Avg ; RETURN ;
+                                 // PRECOND/POWERS/COPY3D: the statement's operands
 };
 
 struct ScalarStmt {
@@ -342,6 +349,16 @@ static int parse_symtab(Record &R, const XNode &root) {
             s.dim0 = attr_i(n, "dim0");
             s.H = 1;
             s.W = (int)s.dim0;
+        } else if (n.tag == "Array3D" && !s.header) {
+            // the HSMatrixPowers result: dim0 planes of dim1 x dim2 (Array3D.cpp:329-341)
+            s.kind = HMX_KIND_ARRAY3D;
+            s.dim0 = attr_i(n, "dim0");
+            s.dim1 = attr_i(n, "dim1");
+            s.dim2 = attr_i(n, "dim2");
+            s.H = (int)s.dim1;
+            s.W = (int)s.dim2;
+            if (s.dim0 < 1 || s.dim1 < 1 || s.dim2 < 1 || s.dim1 > 1 << 30 || s.dim2 > 1 << 30)
+                return fail(HMX_ERR_INVALID, "symbol " + s.name + ": bad dimensions");
         } else if (n.tag == "Scalar") {
             s.kind = HMX_KIND_SCALAR;
         } else if (n.tag == "Stencil") {
@@ -386,7 +403,7 @@ static const std::set<std::string> kFilters = {
     "Array2DOp", "Array2DScalarOp", "ScalarArray2DOp", "Array2DCopy", "Array2DSqrt",
     "Array2DSum", "Stencil2DMVM", "SpmvDIA2D", "SpmvCSR", "ArrayOp", "ArrayScalarOp",
     "ScalarArrayOp", "ArrayCopy", "ArraySqrt", "ArraySum", "ScalarOp", "ScalarCopy",
-    "ScalarSqrt", "Return"};
+    "ScalarSqrt", "Return", "ApplyHSPreconditioner", "HSMatrixPowers", "Array3DCopy"};
 
 static int parse_block(Record &R, const XNode &paf, std::vector<Stmt> &out, int &counter) {
     for (const XNode &n : paf.kids) {
@@ -507,6 +524,53 @@ static int check_stmt(Record &R, const Stmt &st) {
         const Sym &A = R.syms[st.args[1]], &x = R.syms[st.args[2]], &y = R.syms[st.args[0]];
         if (A.dim1 != x.dim0 || A.dim0 != y.dim0)
             return fail(HMX_ERR_INVALID, "SpmvCSR: matrix and vector sizes do not match");
+    } else if (f == "ApplyHSPreconditioner") {
+        // (y0, y1 <- D, diag0, diag1, offdiag, x0, x1), Templates.py:412-430
+        if ((rc = need(8))) return rc;
+        if (K(2) != HMX_KIND_STENCIL)
+            return fail(HMX_ERR_INVALID, "ApplyHSPreconditioner: the first argument must be a "
+                                         "Stencil");
+        if (R.syms[st.args[2]].ndiags < 3)
+            return fail(HMX_ERR_INVALID, "ApplyHSPreconditioner: the stencil needs a weight 2 "
+                                         "(at least 3 taps)");
+        const Sym &x = R.syms[st.args[6]];
+        for (int i : {0, 1, 3, 4, 5, 6, 7}) {
+            const Sym &a = R.syms[st.args[i]];
+            if (a.kind != HMX_KIND_ARRAY2D)
+                return fail(HMX_ERR_INVALID, "ApplyHSPreconditioner: " + a.name +
+                                                 " must be an Array2D");
+            if (a.H != x.H || a.W != x.W)
+                return fail(HMX_ERR_INVALID, "ApplyHSPreconditioner: " + a.name +
+                                                 " differs in shape from " + x.name);
+        }
+    } else if (f == "HSMatrixPowers") {
+        // (V0, V1 <- lam, Ix, Iy, r0, r1), Templates.py:277-300
+        if ((rc = need(7))) return rc;
+        if (K(0) != HMX_KIND_ARRAY3D || K(1) != HMX_KIND_ARRAY3D || K(2) != HMX_KIND_SCALAR)
+            return fail(HMX_ERR_INVALID, "HSMatrixPowers expects (Array3D, Array3D <- Scalar, "
+                                         "Array2D x 4)");
+        const Sym &r = R.syms[st.args[5]], &V = R.syms[st.args[0]], &V1 = R.syms[st.args[1]];
+        for (int i : {3, 4, 5, 6}) {
+            const Sym &a = R.syms[st.args[i]];
+            if (a.kind != HMX_KIND_ARRAY2D || a.H != r.H || a.W != r.W)
+                return fail(HMX_ERR_INVALID, "HSMatrixPowers: " + a.name +
+                                                 " must be an Array2D shaped like " + r.name);
+        }
+        if (V.H != r.H || V.W != r.W || V.dim0 != V1.dim0 || V1.H != r.H || V1.W != r.W)
+            return fail(HMX_ERR_INVALID, "HSMatrixPowers: result planes differ from " + r.name);
+        auto it = R.defines.find("STEPS");
+        if (it == R.defines.end())
+            return fail(HMX_ERR_INVALID, "HSMatrixPowers needs defines['STEPS'] "
+                                         "(HSMatrixPowers.cpp:153)");
+        if (std::strtoll(it->second.c_str(), nullptr, 10) != V.dim0)
+            return fail(HMX_ERR_INVALID, "HSMatrixPowers: result has " + std::to_string(V.dim0) +
+                                             " planes, defines['STEPS'] is " + it->second);
+    } else if (f == "Array3DCopy") {
+        if ((rc = need(2))) return rc;
+        const Sym &a = R.syms[st.args[0]], &b = R.syms[st.args[1]];
+        if (a.kind != HMX_KIND_ARRAY3D || b.kind != HMX_KIND_ARRAY3D || a.dim0 != b.dim0 ||
+            a.H != b.H || a.W != b.W)
+            return fail(HMX_ERR_INVALID, "Array3DCopy operands must be Array3Ds of one shape");
     } else if (f == "ScalarOp") {
         if ((rc = need(3))) return rc;
         if (K(0) != HMX_KIND_SCALAR || K(1) != HMX_KIND_SCALAR || K(2) != HMX_KIND_SCALAR)
@@ -527,7 +591,10 @@ static void collect_reads(Record &R, const std::vector<Stmt> &b) {
             collect_reads(R, st.body);
             continue;
         }
-        size_t first = st.filter == "Return" ? 0 : 1;
+        size_t first = st.filter == "Return"
+                           ? 0
+                           : ((st.filter == "ApplyHSPreconditioner" ||
+                               st.filter == "HSMatrixPowers") ? 2 : 1);
         for (size_t i = first; i < st.args.size(); ++i) R.reads[st.args[i]].push_back(st.index);
     }
 }
@@ -594,6 +661,16 @@ class Planner {
                 if ((rc = flush(out))) return rc;
                 Node n;
                 n.t = Node::CSR;
+                n.syms = st.args;
+                out.push_back(n);
+                continue;
+            }
+            if (f == "ApplyHSPreconditioner" || f == "HSMatrixPowers" || f == "Array3DCopy") {
+                if ((rc = flush(out))) return rc;
+                Node n;
+                n.t = f == "ApplyHSPreconditioner"
+                          ? Node::PRECOND
+                          : (f == "HSMatrixPowers" ? Node::POWERS : Node::COPY3D);
                 n.syms = st.args;
                 out.push_back(n);
                 continue;
@@ -938,7 +1015,8 @@ static void mark_buffers(Record &R, const std::vector<Node> &nodes) {
                 for (int s : g.out_syms) R.syms[s].needs_buffer = true;
                 break;
             }
-            case Node::CSR: case Node::RETURN: case Node::HS:
+            case Node::CSR: case Node::RETURN: case Node::HS: case Node::PRECOND:
+            case Node::POWERS: case Node::COPY3D:
                 for (int s : n.syms) R.syms[s].needs_buffer = true;
                 mark_buffers(R, n.body);
                 break;
@@ -953,6 +1031,7 @@ static void size_symbols(Record &R) {
     for (Sym &s : R.syms) {
         switch (s.kind) {
             case HMX_KIND_ARRAY2D: case HMX_KIND_ARRAY: s.bytes[0] = (int64_t)s.H * s.W * 4; break;
+            case HMX_KIND_ARRAY3D: s.bytes[0] = s.dim0 * s.H * s.W * 4; break;
             case HMX_KIND_SCALAR: s.bytes[0] = dtype_size(s.dtype); break;
             case HMX_KIND_STENCIL:
                 s.bytes[0] = s.bytes[1] = s.bytes[2] = s.ndiags * 4;
@@ -978,6 +1057,9 @@ static int allocate(Record &R) {
                 if (!rc && s.renamed) rc = dalloc(R, &s.alt, s.bytes[0]);
                 break;
             case HMX_KIND_SCALAR: rc = dalloc(R, &s.dev[0], 8); break;
+            case HMX_KIND_ARRAY3D:
+                if (s.needs_buffer) rc = dalloc(R, &s.dev[0], s.bytes[0]);
+                break;
             default:
                 for (int k = 0; k < 3 && !rc; ++k) rc = dalloc(R, &s.dev[k], s.bytes[k]);
                 break;
@@ -1083,6 +1165,34 @@ static int exec_node(Record &R, Node &n) {
                 if (rc) return rc;
             }
             return HMX_OK;
+        case Node::PRECOND: {
+            Sym &y0 = R.syms[n.syms[0]], &y1 = R.syms[n.syms[1]], &D = R.syms[n.syms[2]];
+            const Sym &x = R.syms[n.syms[6]];
+            int rc = launch_hs_precond(R.syms[n.syms[3]].cur_buf(), R.syms[n.syms[4]].cur_buf(),
+                                       R.syms[n.syms[5]].cur_buf(), x.cur_buf(),
+                                       R.syms[n.syms[7]].cur_buf(),
+                                       y0.renamed ? y0.other_buf() : y0.cur_buf(),
+                                       y1.renamed ? y1.other_buf() : y1.cur_buf(), x.H, x.W,
+                                       (const float *)D.dev[0], 0.0f, R.stream);
+            if (!rc && y0.renamed) y0.cur ^= 1;
+            if (!rc && y1.renamed) y1.cur ^= 1;
+            return rc;
+        }
+        case Node::POWERS: {
+            Sym &V0 = R.syms[n.syms[0]], &V1 = R.syms[n.syms[1]], &lam = R.syms[n.syms[2]];
+            const Sym &r = R.syms[n.syms[5]];
+            return launch_hs_powers(R.syms[n.syms[3]].cur_buf(), R.syms[n.syms[4]].cur_buf(),
+                                    r.cur_buf(), R.syms[n.syms[6]].cur_buf(),
+                                    (float *)V0.dev[0], (float *)V1.dev[0], r.H, r.W,
+                                    (int)V0.dim0, lam.dev[0], lam.dtype, 0.0f, R.stream);
+        }
+        case Node::COPY3D: {
+            Sym &d = R.syms[n.syms[0]], &src = R.syms[n.syms[1]];
+            if (d.dev[0] != src.dev[0])
+                HMX_HIP_TRY(hipMemcpyAsync(d.dev[0], src.dev[0], (size_t)d.bytes[0],
+                                           hipMemcpyDeviceToDevice, R.stream));
+            return HMX_OK;
+        }
         case Node::HS: {
             Sym &U = R.syms[n.syms[0]], &V = R.syms[n.syms[1]];
             Taps t;
@@ -1106,11 +1216,13 @@ static int exec_node(Record &R, Node &n) {
             for (size_t i = 0; i < n.syms.size(); ++i) {
                 Sym &s = R.syms[n.syms[i]];
                 if (s.kind != HMX_KIND_ARRAY2D && s.kind != HMX_KIND_ARRAY &&
-                    s.kind != HMX_KIND_SCALAR)
+                    s.kind != HMX_KIND_SCALAR && s.kind != HMX_KIND_ARRAY3D)
                     return fail(HMX_ERR_UNSUPPORTED, "returning " + s.name +
                                                          " (a matrix/stencil) is not supported");
                 R.results[i].resize((size_t)s.bytes[0]);
-                const void *src = s.kind == HMX_KIND_SCALAR ? s.dev[0] : (void *)s.cur_buf();
+                const void *src = (s.kind == HMX_KIND_SCALAR || s.kind == HMX_KIND_ARRAY3D)
+                                      ? s.dev[0]
+                                      : (void *)s.cur_buf();
                 HMX_HIP_TRY(hipMemcpyAsync(R.results[i].data(), src, (size_t)s.bytes[0],
                                            hipMemcpyDeviceToHost, R.stream));
             }
@@ -1167,6 +1279,21 @@ static void describe(const Record &R, const std::vector<Node> &nodes, int depth,
                    << ", den=" << R.syms[n.syms[5]].name << ", S=" << R.syms[n.syms[6]].name
                    << ")\n";
                 break;
+            case Node::PRECOND:
+                os << pad << "hs_precondition (" << R.syms[n.syms[0]].name << ", "
+                   << R.syms[n.syms[1]].name << ") = M^-1 (" << R.syms[n.syms[6]].name << ", "
+                   << R.syms[n.syms[7]].name << "), D=" << R.syms[n.syms[2]].name << "\n";
+                break;
+            case Node::POWERS:
+                os << pad << "hs_matrix_powers x" << R.syms[n.syms[0]].dim0 << " ("
+                   << R.syms[n.syms[0]].name << ", " << R.syms[n.syms[1]].name << ") from ("
+                   << R.syms[n.syms[5]].name << ", " << R.syms[n.syms[6]].name
+                   << "), lam=" << R.syms[n.syms[2]].name << "\n";
+                break;
+            case Node::COPY3D:
+                os << pad << "copy " << R.syms[n.syms[0]].name << " = " << R.syms[n.syms[1]].name
+                   << "\n";
+                break;
             case Node::RETURN:
                 os << pad << "return";
                 for (int s : n.syms) os << " " << R.syms[s].name;
@@ -1189,7 +1316,11 @@ static int check_defines(Record &R, const std::vector<Stmt> &b, bool &need2d, bo
         const bool scalar_only = st.filter.rfind("Scalar", 0) == 0 &&
                                  st.filter.find("Array") == std::string::npos;
         if (st.filter == "Return" || scalar_only) continue;
-        if (st.filter.find("2D") != std::string::npos) need2d = true;
+        // the preconditioned-CG operations launch 2D (ApplyHSPreconditioner.cpp:111-116,
+        // HSMatrixPowers.cpp:147-153)
+        if (st.filter.find("2D") != std::string::npos || st.filter.find("3D") != std::string::npos ||
+            st.filter == "ApplyHSPreconditioner" || st.filter == "HSMatrixPowers")
+            need2d = true;
         else need1d = true;
     }
     return HMX_OK;
@@ -1391,12 +1522,14 @@ extern "C" int hmx_result_info(int rec, int i, int *kind, int *dtype, int *ndim,
     const Sym &s = R->syms[R->returns[i]];
     if (kind) *kind = s.kind;
     if (dtype) *dtype = s.dtype;
-    int nd = s.kind == HMX_KIND_ARRAY2D ? 2 : (s.kind == HMX_KIND_ARRAY ? 1 : 0);
+    int nd = s.kind == HMX_KIND_ARRAY3D
+                 ? 3
+                 : (s.kind == HMX_KIND_ARRAY2D ? 2 : (s.kind == HMX_KIND_ARRAY ? 1 : 0));
     if (ndim) *ndim = nd;
     if (shape) {
         shape[0] = s.dim0;
         shape[1] = s.dim1;
-        shape[2] = s.col_mjr ? 1 : 0;
+        shape[2] = nd == 3 ? s.dim2 : (s.col_mjr ? 1 : 0);
     }
     if (name) *name = s.name.c_str();
     return HMX_OK;

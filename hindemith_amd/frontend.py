@@ -22,6 +22,11 @@ Python syntax          operand kinds                                Filter (Temp
                                                                     Array2DSqrt (:480-529)
 ``x = expr``           per value kind                               Array2DCopy, ArrayCopy,
                                                                     ScalarCopy (:638-758)
+``applyHSPreconditioner(D, diag0, diag1, offdiag, x0, x1)``          ApplyHSPreconditioner
+                       Stencil, Array2D x 5 -> (Array2D, Array2D)   (:412-437)
+``hs_matrix_powers(lam, Ix, Iy, r0, r1)``                           HSMatrixPowers
+                       Scalar, Array2D x 4 -> (Array3D, Array3D)    (:277-300), STEPS planes
+``V = <Array3D>``      Array3D                                      Array3DCopy (:698-730)
 ``for i in range(N)``  N an int literal                             Iterator (:57-71)
 ``return a, b``                                                     Return (:760-780)
 =====================  ===========================================  =====================
@@ -134,6 +139,10 @@ class Array2D(Node):
     pass
 
 
+class Array3D(Node):
+    pass
+
+
 def _xml_escape(v):
     return (str(v).replace("&", "&amp;").replace("<", "&lt;").replace(">", "&gt;")
             .replace('"', "&quot;"))
@@ -208,8 +217,7 @@ def _make_header(symtab, name, obj):
 _BINOPS = {"Add", "Sub", "Mult", "Div"}
 _OUT_OF_SCOPE_CALLS = {
     "warp_img2d": "WarpImg2D", "lk_least_squares": "LKLeastSquares",
-    "hs_matrix_powers": "HSMatrixPowers", "apply_hs_preconditioner": "ApplyHSPreconditioner",
-    "apply_brox_preconditioner": "ApplyBroxPreconditioner", "set_brox_matrix": "SetBroxMatrix",
+    "applyBroxPreconditioner": "ApplyBroxPreconditioner", "set_brox_matrix": "SetBroxMatrix",
     "set_brox_redblack_matrix": "SetBroxRedblackMatrix", "red": "RedArray2D",
     "black": "BlackArray2D", "extract_snapshots": "ExtractSnapshots",
     "cgemm_batch": "CgemmBatch", "solve_batch": "SolveBatch", "gamma_weights": "GammaWeights",
@@ -263,7 +271,7 @@ class Emitter:
                 raise HindemithError(f"assignment to '{name}': it is a {cur.tag}, the value "
                                      f"is a {want}")
             if cls is not Scalar:
-                for key in ("dim0", "dim1", "col_mjr"):
+                for key in ("dim0", "dim1", "dim2", "col_mjr"):
                     if str(cur.properties.get(key)) != str(src.properties.get(key)):
                         raise HindemithError(f"assignment to '{name}': shape/layout differs "
                                              "from the value's")
@@ -357,6 +365,10 @@ class Emitter:
         if fname in _OUT_OF_SCOPE_CALLS:
             raise HindemithError(f"{where}: {_OUT_OF_SCOPE_CALLS[fname]} ({fname}) is not on the "
                                  "Horn-Schunck path and is not supported by this backend")
+        if fname == "applyHSPreconditioner":
+            return self._apply_hs_preconditioner(n, where)
+        if fname == "hs_matrix_powers":
+            return self._hs_matrix_powers(n, where)
         rules = {("sum", "Array"): ("ArraySum", Scalar), ("sum2d", "Array2D"): ("Array2DSum", Scalar),
                  ("sqrt", "Scalar"): ("ScalarSqrt", Scalar), ("sqrt", "Array"): ("ArraySqrt", Array),
                  ("sqrt", "Array2D"): ("Array2DSqrt", Array2D)}
@@ -374,6 +386,65 @@ class Emitter:
         m.children += [newvar, src]
         self.vars[id(n)] = newvar
         self.nodes[id(n)] = self.nodes.get(id(n.args[0]), []) + [m]
+
+    def _call_args(self, n, where, kinds):
+        if len(n.args) != len(kinds) or n.keywords:
+            raise HindemithError(f"{where}: {n.func.id} takes {len(kinds)} positional arguments")
+        out = []
+        for a, want in zip(n.args, kinds):
+            v = self.var_of(a, where)
+            k = self.kind(v, where)
+            if k != want:
+                raise HindemithError(f"{where}: {n.func.id}: expected a {want}, got a {k}")
+            out.append(v)
+        return out
+
+    def _same_shape(self, vs, where):
+        ref = self.symtab[vs[0].properties["name"]].properties
+        for v in vs[1:]:
+            p = self.symtab[v.properties["name"]].properties
+            if any(str(p.get(k)) != str(ref.get(k)) for k in ("dim0", "dim1", "col_mjr")):
+                raise HindemithError(f"{where}: operands differ in shape or layout")
+
+    def _apply_hs_preconditioner(self, n, where):
+        """Templates.py:412-437: (y0, y1) = applyHSPreconditioner(D, diag0, diag1,
+        offdiag, x0, x1); the results are new Array2Ds shaped like x0 and x1."""
+        args = self._call_args(n, where, ["Stencil"] + ["Array2D"] * 5)
+        self._same_shape(args[1:], where)
+        y0 = self.create_copy(Array2D, _unique_name(), args[4])
+        y1 = self.create_copy(Array2D, _unique_name(), args[5])
+        m = Filter("ApplyHSPreconditioner")
+        m.children += [y0, y1] + args
+        self.vars[id(n)] = [y0, y1]
+        self.nodes[id(n)] = list(itertools.chain.from_iterable(
+            self.nodes.get(id(a), []) for a in n.args)) + [m]
+
+    def _hs_matrix_powers(self, n, where):
+        """Templates.py:277-300: (V0, V1) = hs_matrix_powers(lam, Ix, Iy, r0, r1),
+        two Array3Ds of defines['STEPS'] planes shaped like r0."""
+        args = self._call_args(n, where, ["Scalar"] + ["Array2D"] * 4)
+        self._same_shape(args[1:], where)
+        if "STEPS" not in self.defines:
+            raise HindemithError(f"{where}: hs_matrix_powers needs defines['STEPS'] "
+                                 "(HSMatrixPowers.cpp:153)")
+        steps = int(self.defines["STEPS"])
+        if steps < 1:
+            raise HindemithError(f"{where}: defines['STEPS'] must be >= 1")
+        r = self.symtab[args[3].properties["name"]].properties
+        if r.get("col_mjr") == "True":
+            raise HindemithError(f"{where}: hs_matrix_powers needs C-contiguous arrays")
+        outs = []
+        for _ in range(2):
+            nm = _unique_name()
+            self.symtab[nm] = Array3D(nm, source="Inferred", io="inout", dtype=r["dtype"],
+                                      dim0=steps, dim1=r["dim0"], dim2=r["dim1"],
+                                      col_mjr="False", shape=(steps, r["dim0"], r["dim1"]))
+            outs.append(Name(nm))
+        m = Filter("HSMatrixPowers")
+        m.children += outs + args
+        self.vars[id(n)] = outs
+        self.nodes[id(n)] = list(itertools.chain.from_iterable(
+            self.nodes.get(id(a), []) for a in n.args)) + [m]
 
     # -- statements ---------------------------------------------------------------
     def visit_Assign(self, n):
@@ -395,7 +466,8 @@ class Emitter:
                 raise HindemithError(f"{where}: only plain names can be assigned")
             k = self.kind(v, where)
             filt, cls = {"Array2D": ("Array2DCopy", Array2D), "Array": ("ArrayCopy", Array),
-                         "Scalar": ("ScalarCopy", Scalar)}.get(k, (None, None))
+                         "Scalar": ("ScalarCopy", Scalar),
+                         "Array3D": ("Array3DCopy", Array3D)}.get(k, (None, None))
             if filt is None:
                 raise HindemithError(f"{where}: cannot assign a {k}")
             target = self.create_copy(cls, t.id, v)

@@ -13,12 +13,18 @@ places (both documented divergences of the reference itself):
 * ``SpMat2D * x`` reads the coefficient at the neighbour index p+o, as the
   kernel does (SpMat2D.cpp:211-216); the reference's twin reads it at p
   (hlib.py:163).  Both agree for planes that are constant per diagonal.
+* ``applyHSPreconditioner`` passes a block through where |det| <= 1e-15, the
+  kernel's threshold (ApplyHSPreconditioner.cpp:73); the reference's twin uses
+  1e-4 (hlib.py:38).
+* ``hs_matrix_powers`` has no NumPy twin in the reference; this one follows
+  HSMatrixPowers.cpp:72-126 and takes the number of powers from
+  ``defines['STEPS']`` as the kernel does.
 """
 from __future__ import annotations
 
 import numpy as np
 
-__all__ = ["Stencil", "SpMat2D", "sum2d"]
+__all__ = ["Stencil", "SpMat2D", "sum2d", "applyHSPreconditioner", "hs_matrix_powers"]
 
 
 class Stencil:
@@ -77,3 +83,58 @@ class SpMat2D:
 def sum2d(x):
     """Sum of all elements (hlib.py:83-84)."""
     return np.sum(np.asarray(x).flatten())
+
+
+def _neighbour_count(H, W):
+    n = np.zeros((H, W), np.float32)
+    n[:, :-1] += 1
+    n[1:, :] += 1
+    n[:, 1:] += 1
+    n[:-1, :] += 1
+    return n
+
+
+def applyHSPreconditioner(D, diag0, diag1, offdiag, x0, x1):
+    """2x2 block-Jacobi step (ApplyHSPreconditioner.cpp:46-97): solve
+    [[a, b], [b, d]] y = x per pixel with a = diag0 + n*w2, d = diag1 + n*w2,
+    b = offdiag, n the in-image 4-neighbours, w2 = D.data[2]; y = x where
+    |det| <= 1e-15."""
+    x0 = np.asarray(x0)
+    H, W = x0.shape
+    s2 = np.float32(D.data[2])
+    n = _neighbour_count(H, W)
+    a = (diag0 + n * s2).astype(x0.dtype)
+    d = (diag1 + n * s2).astype(x0.dtype)
+    b = np.asarray(offdiag)
+    det = a * d - b * b
+    ok = np.abs(det.astype(np.float64)) > 1e-15
+    safe = np.where(ok, det, 1)
+    y0 = np.where(ok, (x0 * d - b * x1) / safe, x0)
+    y1 = np.where(ok, (-b * x0 + a * x1) / safe, x1)
+    return y0.astype(x0.dtype), y1.astype(x0.dtype)
+
+
+def hs_matrix_powers(lam, Ix, Iy, r0, r1, steps=None):
+    """V[0] = (r0, r1), V[s] = A V[s-1] for s < steps (HSMatrixPowers.cpp:72-126):
+    A = [Ix (Ix u + Iy v), Iy (Ix u + Iy v)] + lam * (4-neighbour Neumann
+    Laplacian).  steps defaults to defines['STEPS'] of the hindemith module."""
+    if steps is None:
+        from .hindemith import defines
+        steps = int(defines["STEPS"])
+    r0 = np.asarray(r0)
+    H, W = r0.shape
+    V0 = np.empty((steps, H, W), r0.dtype)
+    V1 = np.empty_like(V0)
+    V0[0], V1[0] = r0, r1
+    for s in range(1, steps):
+        u, v = V0[s - 1], V1[s - 1]
+        dot = Ix * u + Iy * v
+        U, V = Ix * dot, Iy * dot
+        for nb_u, nb_v, sl in ((u[:, :-1], v[:, :-1], np.s_[:, 1:]),
+                               (u[:, 1:], v[:, 1:], np.s_[:, :-1]),
+                               (u[:-1, :], v[:-1, :], np.s_[1:, :]),
+                               (u[1:, :], v[1:, :], np.s_[:-1, :])):
+            U[sl] += lam * (u[sl] - nb_u)
+            V[sl] += lam * (v[sl] - nb_v)
+        V0[s], V1[s] = U, V
+    return V0, V1

@@ -19,6 +19,7 @@ from ._lib import HindemithError, check, lib
 __all__ = [
     "taps_of", "stencil2d", "dia2d_spmv", "csr_spmv", "CSRBins", "csr_spmv_binned",
     "sum2d", "hs_setup", "hs_jacobi", "hs_solve", "hs_tune", "hs_variants", "stream_ptr",
+    "hs_precondition", "hs_matrix_powers",
 ]
 
 
@@ -220,3 +221,44 @@ def hs_solve(I1, I2, alpha2, taps, iters, u=None, v=None, workspace=None):
                              int(iters), _ptr(u), _ptr(v), _ptr(workspace), stream_ptr()),
           "hs_solve")
     return u, v
+
+
+def hs_precondition(D, diag0, diag1, offdiag, x0, x1, out=None):
+    """a15: (y0, y1) = M^-1 (x0, x1), the 2x2 block-Jacobi step of the
+    reference's applyHSPreconditioner(D, diag0, diag1, offdiag, x0, x1); D is the
+    stencil whose weight 2 enters the diagonal blocks."""
+    for n, t in (("diag0", diag0), ("diag1", diag1), ("offdiag", offdiag), ("x0", x0),
+                 ("x1", x1)):
+        _dev(t, n)
+    H, W = x0.shape
+    for t in (diag0, diag1, offdiag, x1):
+        if tuple(t.shape) != (H, W):
+            raise HindemithError("hs_precondition: all planes must have the same shape")
+    taps = _Taps(D)
+    y0, y1 = (torch.empty_like(x0), torch.empty_like(x0)) if out is None else out
+    check(lib().hmx_hs_precondition(_ptr(diag0), _ptr(diag1), _ptr(offdiag), _ptr(x0), _ptr(x1),
+                                    _ptr(_dev(y0, "y0")), _ptr(_dev(y1, "y1")), H, W,
+                                    len(taps.w), taps.w.ctypes.data_as(ctypes.c_void_p),
+                                    stream_ptr()), "hs_precondition")
+    return y0, y1
+
+
+def hs_matrix_powers(lam, Ix, Iy, r0, r1, steps, out=None):
+    """a16: V[0] = (r0, r1), V[s] = A V[s-1] for s < steps (the reference's
+    hs_matrix_powers(lam, Ix, Iy, r0, r1) with defines['STEPS'] = steps);
+    returns (V0, V1), each steps x H x W."""
+    for n, t in (("Ix", Ix), ("Iy", Iy), ("r0", r0), ("r1", r1)):
+        _dev(t, n)
+    H, W = r0.shape
+    for t in (Ix, Iy, r1):
+        if tuple(t.shape) != (H, W):
+            raise HindemithError("hs_matrix_powers: all planes must have the same shape")
+    if out is None:
+        V0 = torch.empty((steps, H, W), device=r0.device, dtype=torch.float32)
+        V1 = torch.empty_like(V0)
+    else:
+        V0, V1 = (_dev(t, "V") for t in out)
+    check(lib().hmx_hs_matrix_powers(ctypes.c_float(lam), _ptr(Ix), _ptr(Iy), _ptr(r0),
+                                     _ptr(r1), _ptr(V0), _ptr(V1), H, W, int(steps),
+                                     stream_ptr()), "hs_matrix_powers")
+    return V0, V1

@@ -47,7 +47,8 @@ enum hmx_kind {
     HMX_KIND_SCALAR = 3,
     HMX_KIND_STENCIL = 4,
     HMX_KIND_SPMAT2D = 5,
-    HMX_KIND_CSRMAT = 6
+    HMX_KIND_CSRMAT = 6,
+    HMX_KIND_ARRAY3D = 7 /* only as the HSMatrixPowers result (dim0 = STEPS planes) */
 };
 
 const char *hmx_last_error(void);
@@ -82,7 +83,8 @@ int hmx_bind(int rec, const char *name, int component, const void *host, int64_t
 int hmx_run(int rec);
 
 int hmx_result_count(int rec, int *n);
-/* shape has room for 3 entries; *name stays valid until hmx_release. */
+/* shape has room for 3 entries: ndim 1 or 2 -> (dim0, dim1, col_mjr flag);
+ * ndim 3 -> (dim0, dim1, dim2), C order.  *name stays valid until hmx_release. */
 int hmx_result_info(int rec, int i, int *kind, int *dtype, int *ndim, int64_t *shape,
                     const char **name);
 /* Copy return value i of the last run into host memory (createPyObject,
@@ -303,6 +305,27 @@ int hmx_hs_batched_resident(int variant, int ntaps, int *api_per_cu, int *own_pe
                             int *cus);
 /* Free every cached batched launch (descriptors, graphs). */
 int hmx_hs_batched_release(void);
+
+/* ---------------------------------------------------------------------------
+ * Preconditioned-CG Horn-Schunck operations (device pointers, fp32, H x W
+ * row-major planes, the caller's stream).
+ */
+
+/* a15. (y0, y1) = M^-1 (x0, x1) per pixel, M = [[a, b], [b, d]] with
+ * a = diag0 + n*w[2], d = diag1 + n*w[2], b = offdiag, n the number of in-image
+ * 4-neighbours; where |det M| <= 1e-15, y = x.  w: the host weights of the
+ * stencil passed to applyHSPreconditioner (nd >= 3).
+ * Replaces ApplyHSPreconditioner (ApplyHSPreconditioner.cpp:46-97). */
+int hmx_hs_precondition(const float *diag0, const float *diag1, const float *offdiag,
+                        const float *x0, const float *x1, float *y0, float *y1, int H, int W,
+                        int nd, const float *w, void *stream);
+/* a16. V0, V1 (steps planes of H x W each): V[0] = (r0, r1), V[s] = A V[s-1],
+ * A the HS operator Ix(Ix u + Iy v) + lam*L u (and the v twin) with L the
+ * 4-neighbour Neumann Laplacian.  Replaces HSMatrixPowers
+ * (HSMatrixPowers.cpp:28-39, 72-126, 191-226). */
+int hmx_hs_matrix_powers(float lam, const float *Ix, const float *Iy, const float *r0,
+                         const float *r1, float *V0, float *V1, int H, int W, int steps,
+                         void *stream);
 
 #ifdef __cplusplus
 }

@@ -438,3 +438,137 @@ void or_hs_solve_f32(const float *I1, const float *I2, int H, int W, float alpha
     or_hs_jacobi_f32(Ix, Iy, It, alpha2, H, W, nda, wa, oxa, oya, iters, u, v);
     free(Ix); free(Iy); free(It);
 }
+
+/* ------------------------------------------------------------------------
+ * Preconditioned-CG Horn-Schunck operations.
+ *
+ * or_hs_precond_*: ApplyHSPreconditioner.cpp:46-97.  Per pixel, with s2 the
+ * stencil's weight 2: a = diag0 + s2 per in-image 4-neighbour (added in the
+ * order x<W-1, y<H-1, x>0, y>0), d likewise from diag1, b = c = offdiag;
+ * det = a*d - b*c; if |det| > 1e-15 (a double literal):
+ * y0 = (e*d - b*f)/det, y1 = (-c*e + a*f)/det, else y = (e, f).
+ */
+void or_hs_precond_f32(const float *diag0, const float *diag1, const float *offdiag,
+                       const float *x0, const float *x1, float *y0, float *y1, int H, int W,
+                       float s2)
+{
+#pragma omp parallel for schedule(static)
+    for (int r = 0; r < H; ++r)
+        for (int c = 0; c < W; ++c) {
+            const int64_t p = (int64_t)r * W + c;
+            float a = diag0[p], d = diag1[p];
+            if (c < W - 1) { a = a + s2; d = d + s2; }
+            if (r < H - 1) { a = a + s2; d = d + s2; }
+            if (c > 0) { a = a + s2; d = d + s2; }
+            if (r > 0) { a = a + s2; d = d + s2; }
+            const float b = offdiag[p], e = x0[p], f = x1[p];
+            const float ad = a * d, bc = b * b;
+            const float det = ad - bc;
+            if (fabs((double)det) > 1e-15) {
+                const float t0 = e * d, t1 = b * f;
+                const float n0 = t0 - t1;
+                const float t2 = (-b) * e, t3 = a * f;
+                const float n1 = t2 + t3;
+                y0[p] = n0 / det;
+                y1[p] = n1 / det;
+            } else {
+                y0[p] = e;
+                y1[p] = f;
+            }
+        }
+}
+
+void or_hs_precond_f64(const double *diag0, const double *diag1, const double *offdiag,
+                       const double *x0, const double *x1, double *y0, double *y1, int H, int W,
+                       double s2)
+{
+#pragma omp parallel for schedule(static)
+    for (int r = 0; r < H; ++r)
+        for (int c = 0; c < W; ++c) {
+            const int64_t p = (int64_t)r * W + c;
+            double a = diag0[p], d = diag1[p];
+            if (c < W - 1) { a += s2; d += s2; }
+            if (r < H - 1) { a += s2; d += s2; }
+            if (c > 0) { a += s2; d += s2; }
+            if (r > 0) { a += s2; d += s2; }
+            const double b = offdiag[p], e = x0[p], f = x1[p];
+            const double det = a * d - b * b;
+            if (fabs(det) > 1e-15) {
+                y0[p] = (e * d - b * f) / det;
+                y1[p] = (-b * e + a * f) / det;
+            } else {
+                y0[p] = e;
+                y1[p] = f;
+            }
+        }
+}
+
+/*
+ * or_hs_powers_*: HSMatrixPowers.cpp:28-39 (V[0] = r), :72-126 (one power)
+ * and :206-226 (s = 1..steps-1).  V0, V1 hold steps planes of H x W.  Per
+ * pixel of power s from (u, v) = power s-1:
+ *   U = Ix * (Ix*u + Iy*v);  V = Iy * (Ix*u + Iy*v)
+ *   then for each in-image neighbour, in the order left, right, up, down:
+ *   U += -lam * u_nb;  V += -lam * v_nb;  U += lam * u;  V += lam * v
+ * (the "+=" statements as fmas in the fp32 restatement).
+ */
+void or_hs_powers_f32(float lam, const float *Ix, const float *Iy, const float *r0,
+                      const float *r1, float *V0, float *V1, int H, int W, int steps)
+{
+    const int64_t n = (int64_t)H * W;
+    memcpy(V0, r0, sizeof(float) * n);
+    memcpy(V1, r1, sizeof(float) * n);
+    const float nlam = -lam;
+    for (int s = 1; s < steps; ++s) {
+        const float *u = V0 + (int64_t)(s - 1) * n, *v = V1 + (int64_t)(s - 1) * n;
+        float *U = V0 + (int64_t)s * n, *V = V1 + (int64_t)s * n;
+#pragma omp parallel for schedule(static)
+        for (int r = 0; r < H; ++r)
+            for (int c = 0; c < W; ++c) {
+                const int64_t p = (int64_t)r * W + c;
+                const float ix = Ix[p], iy = Iy[p], um = u[p], vm = v[p];
+                const float p1 = ix * um, p2 = iy * vm;
+                const float dot = p1 + p2;
+                float ur = ix * dot, vr = iy * dot;
+                const int64_t nb[4] = {p - 1, p + 1, p - W, p + W};
+                const int ok[4] = {c > 0, c < W - 1, r > 0, r < H - 1};
+                for (int k = 0; k < 4; ++k)
+                    if (ok[k]) {
+                        ur = fmaf(nlam, u[nb[k]], ur);
+                        vr = fmaf(nlam, v[nb[k]], vr);
+                        ur = fmaf(lam, um, ur);
+                        vr = fmaf(lam, vm, vr);
+                    }
+                U[p] = ur;
+                V[p] = vr;
+            }
+    }
+}
+
+void or_hs_powers_f64(double lam, const double *Ix, const double *Iy, const double *r0,
+                      const double *r1, double *V0, double *V1, int H, int W, int steps)
+{
+    const int64_t n = (int64_t)H * W;
+    memcpy(V0, r0, sizeof(double) * n);
+    memcpy(V1, r1, sizeof(double) * n);
+    for (int s = 1; s < steps; ++s) {
+        const double *u = V0 + (int64_t)(s - 1) * n, *v = V1 + (int64_t)(s - 1) * n;
+        double *U = V0 + (int64_t)s * n, *V = V1 + (int64_t)s * n;
+#pragma omp parallel for schedule(static)
+        for (int r = 0; r < H; ++r)
+            for (int c = 0; c < W; ++c) {
+                const int64_t p = (int64_t)r * W + c;
+                const double dot = Ix[p] * u[p] + Iy[p] * v[p];
+                double ur = Ix[p] * dot, vr = Iy[p] * dot;
+                const int64_t nb[4] = {p - 1, p + 1, p - W, p + W};
+                const int ok[4] = {c > 0, c < W - 1, r > 0, r < H - 1};
+                for (int k = 0; k < 4; ++k)
+                    if (ok[k]) {
+                        ur += lam * (u[p] - u[nb[k]]);
+                        vr += lam * (v[p] - v[nb[k]]);
+                    }
+                U[p] = ur;
+                V[p] = vr;
+            }
+    }
+}

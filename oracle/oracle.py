@@ -15,7 +15,7 @@ __all__ = [
     "build_oracle", "load_oracle", "stencil2d", "stencil2d_T", "dia2d", "csr_spmv",
     "sum2d", "hs_setup", "hs_jacobi", "hs_solve", "hs_grad", "num_threads",
     "set_num_threads", "np_stencil2d", "np_dia2d_csr", "np_hs_solve_f64",
-    "endpoint_error",
+    "endpoint_error", "hs_precondition", "hs_matrix_powers",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -255,6 +255,41 @@ def np_hs_solve_f64(I1, I2, alpha2, taps, iters, u=None, v=None):
         u = ub - Ix * t
         v = vb - Iy * t
     return u, v
+
+
+def hs_precondition(s2, diag0, diag1, offdiag, x0, x1, dtype=np.float32):
+    """ApplyHSPreconditioner.cpp:46-97: (y0, y1) = M^-1 (x0, x1) per pixel; s2 is
+    weight 2 of the stencil operand."""
+    lib = load_oracle()
+    cast = _f32 if dtype == np.float32 else _f64
+    d0, d1, od, a, b = (cast(t) for t in (diag0, diag1, offdiag, x0, x1))
+    H, W = a.shape
+    y0, y1 = np.empty_like(a), np.empty_like(a)
+    if dtype == np.float32:
+        lib.or_hs_precond_f32(_p(d0), _p(d1), _p(od), _p(a), _p(b), _p(y0), _p(y1), H, W,
+                              ctypes.c_float(s2))
+    else:
+        lib.or_hs_precond_f64(_p(d0), _p(d1), _p(od), _p(a), _p(b), _p(y0), _p(y1), H, W,
+                              ctypes.c_double(s2))
+    return y0, y1
+
+
+def hs_matrix_powers(lam, Ix, Iy, r0, r1, steps, dtype=np.float32):
+    """HSMatrixPowers.cpp:28-39, 72-126, 206-226: V[0] = r, V[s] = A V[s-1];
+    returns (V0, V1), each steps x H x W."""
+    lib = load_oracle()
+    cast = _f32 if dtype == np.float32 else _f64
+    ix, iy, a, b = (cast(t) for t in (Ix, Iy, r0, r1))
+    H, W = a.shape
+    V0 = np.empty((steps, H, W), dtype=dtype)
+    V1 = np.empty_like(V0)
+    if dtype == np.float32:
+        lib.or_hs_powers_f32(ctypes.c_float(lam), _p(ix), _p(iy), _p(a), _p(b), _p(V0), _p(V1),
+                             H, W, int(steps))
+    else:
+        lib.or_hs_powers_f64(ctypes.c_double(lam), _p(ix), _p(iy), _p(a), _p(b), _p(V0),
+                             _p(V1), H, W, int(steps))
+    return V0, V1
 
 
 def endpoint_error(u, v, fx, fy, margin=8):

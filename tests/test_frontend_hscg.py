@@ -1,0 +1,91 @@
+"""Front end and planner for the preconditioned-CG Horn-Schunck operations
+(applyHSPreconditioner, hs_matrix_powers), on the CPU (plan-only mode), and
+the NumPy twins of both against the oracle."""
+import numpy as np
+import pytest
+
+import oracle
+from hindemith_amd import frontend as fe
+from hindemith_amd import synthetic as S
+from hindemith_amd._lib import HindemithError
+from hindemith_amd.hindemith import (Stencil, applyHSPreconditioner, defines, hm,
+                                     hs_matrix_powers)
+
+DEF2D = dict(nd20=1, nd21=1, nt20=8, nt21=8, ne20=2, ne21=2, STEPS=4)
+
+
+@pytest.fixture(autouse=True)
+def plan_only():
+    fe.set_plan_only(True)
+    saved = dict(defines)
+    defines.clear()
+    defines.update(DEF2D)
+    yield
+    defines.clear()
+    defines.update(saved)
+    fe.set_plan_only(False)
+
+
+@hm
+def cg_step(D, Ix, Iy, diag0, diag1, offdiag, r0, r1, lam):
+    z0, z1 = applyHSPreconditioner(D, diag0, diag1, offdiag, r0, r1)
+    V0, V1 = hs_matrix_powers(lam, Ix, Iy, z0, z1)
+    return V0, V1, z0
+
+
+def kw(H=24, W=40):
+    rng = np.random.default_rng(0)
+    a = [rng.normal(size=(H, W)).astype(np.float32) for _ in range(7)]
+    return dict(D=Stencil(*S.laplacian5_taps()), Ix=a[0], Iy=a[1], diag0=a[2], diag1=a[3],
+                offdiag=a[4], r0=a[5], r1=a[6], lam=np.float32(0.5))
+
+
+def test_plan_has_both_operations():
+    plan = cg_step.compile(**kw())
+    assert "hs_precondition (" in plan
+    assert "hs_matrix_powers x4" in plan
+    assert plan.strip().splitlines()[-1].startswith("return ")
+
+
+def test_xml_names_the_reference_filters():
+    rec = fe.mycompile(cg_step.source, kw(), defines, fn_key=("xml", cg_step.source))
+    assert 'Filter name="ApplyHSPreconditioner"' in rec.model_xml
+    assert 'Filter name="HSMatrixPowers"' in rec.model_xml
+    assert "<Array3D " in rec.symtab_xml and 'dim0="4"' in rec.symtab_xml
+
+
+def test_steps_define_required():
+    defines.pop("STEPS")
+    with pytest.raises(HindemithError, match="STEPS"):
+        cg_step.compile(**kw(8, 8))
+
+
+def test_wrong_operand_kinds():
+    @hm
+    def bad(x, y, lam):
+        V0, V1 = hs_matrix_powers(lam, x, y, x, lam)
+        return V0
+    z = np.zeros((4, 4), np.float32)
+    with pytest.raises(HindemithError, match="expected a Array2D, got a Scalar"):
+        bad.compile(x=z, y=z, lam=np.float32(1.0))
+
+
+def test_shape_mismatch():
+    k = kw()
+    k["diag1"] = np.zeros((3, 3), np.float32)
+    with pytest.raises(HindemithError, match="shape"):
+        cg_step.compile(**k)
+
+
+def test_numpy_twins_follow_the_oracle():
+    k = kw(9, 13)
+    D = k["D"]
+    y0, y1 = applyHSPreconditioner(D, k["diag0"], k["diag1"], k["offdiag"], k["r0"], k["r1"])
+    r0, r1 = oracle.hs_precondition(float(D.data[2]), k["diag0"], k["diag1"], k["offdiag"],
+                                    k["r0"], k["r1"], np.float64)
+    assert np.allclose(y0, r0, rtol=1e-4, atol=1e-4) and np.allclose(y1, r1, rtol=1e-4, atol=1e-4)
+    V0, V1 = hs_matrix_powers(0.5, k["Ix"].astype(np.float64), k["Iy"].astype(np.float64),
+                              k["r0"].astype(np.float64), k["r1"].astype(np.float64))
+    R0, R1 = oracle.hs_matrix_powers(0.5, k["Ix"], k["Iy"], k["r0"], k["r1"], 4, np.float64)
+    assert np.allclose(V0, R0, rtol=1e-12, atol=1e-12) and np.allclose(V1, R1, rtol=1e-12,
+                                                                         atol=1e-12)

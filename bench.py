@@ -263,8 +263,9 @@ def extra_batched(S):
 
 
 def extra_hscg(K, S):
-    """The preconditioned-CG operations at 1920x1080 fp32, timed as hipGraph
-    replays: the 2x2 block-Jacobi preconditioner (28 B/px: read diag0, diag1,
+    """The preconditioned-CG operations and the image warp at 1920x1080 fp32,
+    timed as hipGraph replays: the warp of frame 2 by the synthetic flow (16 B/px:
+    read src, offx, offy; write dst), the 2x2 block-Jacobi preconditioner (28 B/px: read diag0, diag1,
     offdiag, x0, x1; write y0, y1) and 8 HS operator powers (algorithmic bytes
     as the reference's one launch per power: V[0] copy 16 B/px + 24 B/px per
     power = 184 B/px; the blocked kernel's minimum traffic is 88 B/px: read Ix,
@@ -285,7 +286,13 @@ def extra_hscg(K, S):
     ms_m = _time_graph_ms(lambda: K.hs_matrix_powers(1.0, Ix, Iy, r0, r1, steps, out=V),
                           launches=10)
     alg = (16 + 24 * (steps - 1)) * n
+    fx, fy = (torch.as_tensor(f, device="cuda", dtype=torch.float32).contiguous()
+              for f in S.affine_flow(H, W))
+    wdst = torch.empty_like(r0)
+    I2d = torch.from_numpy(I2).cuda()
+    ms_w = _time_graph_ms(lambda: K.warp_img2d(I2d, fx, fy, out=wdst))
     return {"workload": f"{W}x{H} fp32",
+            "warp_img2d": {"us": round(ms_w * 1e3, 2), "GBps": round(16 * n / ms_w / 1e6, 1)},
             "precondition": {"us": round(ms_p * 1e3, 2), "GBps": round(28 * n / ms_p / 1e6, 1)},
             "matrix_powers_8": {"us": round(ms_m * 1e3, 2),
                                 "Mpx_power_per_s": round(n * (steps - 1) / ms_m / 1e3, 1),

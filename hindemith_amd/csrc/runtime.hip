@@ -47,6 +47,8 @@ int launch_hs_precond(const float *diag0, const float *diag1, const float *offdi
 int launch_hs_powers(const float *Ix, const float *Iy, const float *r0, const float *r1,
                      float *V0, float *V1, int H, int W, int steps, const void *lam_dev,
                      int lam_dtype, float lam_val, hipStream_t s);
+int launch_warp_img2d(const float *src, const float *offx, const float *offy, float *dst, int H,
+                      int W, hipStream_t s);
 
 namespace rt {
 
@@ -272,12 +274,12 @@ struct Group {
 };
 
 struct Node {
-    enum T { GROUP, SCALAR, CSR, LOOP, HS, RETURN, PRECOND, POWERS, COPY3D } t;
+    enum T { GROUP, SCALAR, CSR, LOOP, HS, RETURN, PRECOND, POWERS, COPY3D, WARP } t;
     int idx = 0;                 // GROUP: groups[], SCALAR: sprogs[]
     int ub = 0;                  // LOOP / HS
     std::vector<Node> body;      // LOOP (and HS fallback)
     std::vector<int> syms;       // CSR: y, A, x ; HS: u, v, Ix, Iy, It, den, Avg ; RETURN ;
-                                 // PRECOND/POWERS/COPY3D: the statement's operands
+                                 // PRECOND/POWERS/COPY3D/WARP: the statement's operands
 };
 
 struct ScalarStmt {
@@ -403,7 +405,8 @@ static const std::set<std::string> kFilters = {
     "Array2DOp", "Array2DScalarOp", "ScalarArray2DOp", "Array2DCopy", "Array2DSqrt",
     "Array2DSum", "Stencil2DMVM", "SpmvDIA2D", "SpmvCSR", "ArrayOp", "ArrayScalarOp",
     "ScalarArrayOp", "ArrayCopy", "ArraySqrt", "ArraySum", "ScalarOp", "ScalarCopy",
-    "ScalarSqrt", "Return", "ApplyHSPreconditioner", "HSMatrixPowers", "Array3DCopy"};
+    "ScalarSqrt", "Return", "ApplyHSPreconditioner", "HSMatrixPowers", "Array3DCopy",
+    "WarpImg2D"};
 
 static int parse_block(Record &R, const XNode &paf, std::vector<Stmt> &out, int &counter) {
     for (const XNode &n : paf.kids) {
@@ -565,6 +568,16 @@ static int check_stmt(Record &R, const Stmt &st) {
         if (std::strtoll(it->second.c_str(), nullptr, 10) != V.dim0)
             return fail(HMX_ERR_INVALID, "HSMatrixPowers: result has " + std::to_string(V.dim0) +
                                              " planes, defines['STEPS'] is " + it->second);
+    } else if (f == "WarpImg2D") {
+        // (dst <- src, offx, offy), Templates.py:257-274
+        if ((rc = need(4))) return rc;
+        const Sym &x = R.syms[st.args[1]];
+        for (int i = 0; i < 4; ++i) {
+            const Sym &a = R.syms[st.args[i]];
+            if (a.kind != HMX_KIND_ARRAY2D || a.H != x.H || a.W != x.W || a.col_mjr != x.col_mjr)
+                return fail(HMX_ERR_INVALID, "WarpImg2D: " + a.name +
+                                                 " must be an Array2D shaped like " + x.name);
+        }
     } else if (f == "Array3DCopy") {
         if ((rc = need(2))) return rc;
         const Sym &a = R.syms[st.args[0]], &b = R.syms[st.args[1]];
@@ -665,12 +678,15 @@ class Planner {
                 out.push_back(n);
                 continue;
             }
-            if (f == "ApplyHSPreconditioner" || f == "HSMatrixPowers" || f == "Array3DCopy") {
+            if (f == "ApplyHSPreconditioner" || f == "HSMatrixPowers" || f == "Array3DCopy" ||
+                f == "WarpImg2D") {
                 if ((rc = flush(out))) return rc;
                 Node n;
                 n.t = f == "ApplyHSPreconditioner"
                           ? Node::PRECOND
-                          : (f == "HSMatrixPowers" ? Node::POWERS : Node::COPY3D);
+                          : (f == "HSMatrixPowers"
+                                 ? Node::POWERS
+                                 : (f == "WarpImg2D" ? Node::WARP : Node::COPY3D));
                 n.syms = st.args;
                 out.push_back(n);
                 continue;
@@ -1016,7 +1032,7 @@ static void mark_buffers(Record &R, const std::vector<Node> &nodes) {
                 break;
             }
             case Node::CSR: case Node::RETURN: case Node::HS: case Node::PRECOND:
-            case Node::POWERS: case Node::COPY3D:
+            case Node::POWERS: case Node::COPY3D: case Node::WARP:
                 for (int s : n.syms) R.syms[s].needs_buffer = true;
                 mark_buffers(R, n.body);
                 break;
@@ -1186,6 +1202,15 @@ static int exec_node(Record &R, Node &n) {
                                     (float *)V0.dev[0], (float *)V1.dev[0], r.H, r.W,
                                     (int)V0.dim0, lam.dev[0], lam.dtype, 0.0f, R.stream);
         }
+        case Node::WARP: {
+            Sym &d = R.syms[n.syms[0]], &src = R.syms[n.syms[1]];
+            int rc = launch_warp_img2d(src.cur_buf(), R.syms[n.syms[2]].cur_buf(),
+                                       R.syms[n.syms[3]].cur_buf(),
+                                       d.renamed ? d.other_buf() : d.cur_buf(), src.H, src.W,
+                                       R.stream);
+            if (!rc && d.renamed) d.cur ^= 1;
+            return rc;
+        }
         case Node::COPY3D: {
             Sym &d = R.syms[n.syms[0]], &src = R.syms[n.syms[1]];
             if (d.dev[0] != src.dev[0])
@@ -1289,6 +1314,11 @@ static void describe(const Record &R, const std::vector<Node> &nodes, int depth,
                    << R.syms[n.syms[0]].name << ", " << R.syms[n.syms[1]].name << ") from ("
                    << R.syms[n.syms[5]].name << ", " << R.syms[n.syms[6]].name
                    << "), lam=" << R.syms[n.syms[2]].name << "\n";
+                break;
+            case Node::WARP:
+                os << pad << "warp_img2d " << R.syms[n.syms[0]].name << " = "
+                   << R.syms[n.syms[1]].name << " at +(" << R.syms[n.syms[2]].name << ", "
+                   << R.syms[n.syms[3]].name << ")\n";
                 break;
             case Node::COPY3D:
                 os << pad << "copy " << R.syms[n.syms[0]].name << " = " << R.syms[n.syms[1]].name

@@ -27,6 +27,7 @@ Python syntax          operand kinds                                Filter (Temp
 ``hs_matrix_powers(lam, Ix, Iy, r0, r1)``                           HSMatrixPowers
                        Scalar, Array2D x 4 -> (Array3D, Array3D)    (:277-300), STEPS planes
 ``V = <Array3D>``      Array3D                                      Array3DCopy (:698-730)
+``warp_img2d(src, ox, oy)`` Array2D x 3 -> Array2D                  WarpImg2D (:257-274)
 ``for i in range(N)``  N an int literal                             Iterator (:57-71)
 ``return a, b``                                                     Return (:760-780)
 =====================  ===========================================  =====================
@@ -216,7 +217,7 @@ def _make_header(symtab, name, obj):
 
 _BINOPS = {"Add", "Sub", "Mult", "Div"}
 _OUT_OF_SCOPE_CALLS = {
-    "warp_img2d": "WarpImg2D", "lk_least_squares": "LKLeastSquares",
+    "lk_least_squares": "LKLeastSquares",
     "applyBroxPreconditioner": "ApplyBroxPreconditioner", "set_brox_matrix": "SetBroxMatrix",
     "set_brox_redblack_matrix": "SetBroxRedblackMatrix", "red": "RedArray2D",
     "black": "BlackArray2D", "extract_snapshots": "ExtractSnapshots",
@@ -369,6 +370,8 @@ class Emitter:
             return self._apply_hs_preconditioner(n, where)
         if fname == "hs_matrix_powers":
             return self._hs_matrix_powers(n, where)
+        if fname == "warp_img2d":
+            return self._warp_img2d(n, where)
         rules = {("sum", "Array"): ("ArraySum", Scalar), ("sum2d", "Array2D"): ("Array2DSum", Scalar),
                  ("sqrt", "Scalar"): ("ScalarSqrt", Scalar), ("sqrt", "Array"): ("ArraySqrt", Array),
                  ("sqrt", "Array2D"): ("Array2DSqrt", Array2D)}
@@ -416,6 +419,18 @@ class Emitter:
         m = Filter("ApplyHSPreconditioner")
         m.children += [y0, y1] + args
         self.vars[id(n)] = [y0, y1]
+        self.nodes[id(n)] = list(itertools.chain.from_iterable(
+            self.nodes.get(id(a), []) for a in n.args)) + [m]
+
+    def _warp_img2d(self, n, where):
+        """Templates.py:257-274: dst = warp_img2d(src, offx, offy), a new Array2D
+        shaped like src."""
+        args = self._call_args(n, where, ["Array2D"] * 3)
+        self._same_shape(args, where)
+        dst = self.create_copy(Array2D, _unique_name(), args[0])
+        m = Filter("WarpImg2D")
+        m.children += [dst] + args
+        self.vars[id(n)] = dst
         self.nodes[id(n)] = list(itertools.chain.from_iterable(
             self.nodes.get(id(a), []) for a in n.args)) + [m]
 

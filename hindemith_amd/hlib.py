@@ -19,12 +19,17 @@ places (both documented divergences of the reference itself):
 * ``hs_matrix_powers`` has no NumPy twin in the reference; this one follows
   HSMatrixPowers.cpp:72-126 and takes the number of powers from
   ``defines['STEPS']`` as the kernel does.
+* ``warp_img2d`` splits the offset as the kernel does (truncate, one step down
+  when negative, WarpImg2D.cpp:37-50) and copies the source pixel when a tap
+  leaves the image; the reference's twin scales the fractions by 1 - sign
+  (hlib.py:128-129) and tests other bounds.
 """
 from __future__ import annotations
 
 import numpy as np
 
-__all__ = ["Stencil", "SpMat2D", "sum2d", "applyHSPreconditioner", "hs_matrix_powers"]
+__all__ = ["Stencil", "SpMat2D", "sum2d", "applyHSPreconditioner", "hs_matrix_powers",
+           "warp_img2d"]
 
 
 class Stencil:
@@ -138,3 +143,24 @@ def hs_matrix_powers(lam, Ix, Iy, r0, r1, steps=None):
             V[sl] += lam * (v[sl] - nb_v)
         V0[s], V1[s] = U, V
     return V0, V1
+
+
+def warp_img2d(src, offx, offy):
+    """Bilinear warp (WarpImg2D.cpp:33-75): src sampled at (x + offx, y + offy)
+    where the four taps are inside the image, else src(x, y)."""
+    src = np.asarray(src)
+    H, W = src.shape
+    ox, oy = np.asarray(offx), np.asarray(offy)
+    mx, my = np.trunc(ox).astype(np.int64), np.trunc(oy).astype(np.int64)
+    xf, yf = ox - mx, oy - my
+    neg_x, neg_y = ox < 0, oy < 0
+    mx, my = mx - neg_x, my - neg_y
+    xf, yf = np.where(neg_x, 1 + xf, xf), np.where(neg_y, 1 + yf, yf)
+    rr, cc = np.meshgrid(np.arange(H), np.arange(W), indexing="ij")
+    sx, sy = cc + mx, rr + my
+    ok = (sx >= 0) & (sx + 1 < W) & (sy >= 0) & (sy + 1 < H)
+    sx, sy = np.where(ok, sx, 0), np.where(ok, sy, 0)
+    sx1, sy1 = np.minimum(sx + 1, W - 1), np.minimum(sy + 1, H - 1)
+    val = (src[sy, sx] * (1 - xf) * (1 - yf) + src[sy, sx1] * xf * (1 - yf) +
+           src[sy1, sx] * (1 - xf) * yf + src[sy1, sx1] * xf * yf)
+    return np.where(ok, val, src).astype(src.dtype)

@@ -19,7 +19,7 @@ from ._lib import HindemithError, check, lib
 __all__ = [
     "taps_of", "stencil2d", "dia2d_spmv", "csr_spmv", "CSRBins", "csr_spmv_binned",
     "sum2d", "hs_setup", "hs_jacobi", "hs_solve", "hs_tune", "hs_variants", "stream_ptr",
-    "hs_precondition", "hs_matrix_powers",
+    "hs_precondition", "hs_matrix_powers", "warp_img2d",
 ]
 
 
@@ -262,3 +262,17 @@ def hs_matrix_powers(lam, Ix, Iy, r0, r1, steps, out=None):
                                      _ptr(r1), _ptr(V0), _ptr(V1), H, W, int(steps),
                                      stream_ptr()), "hs_matrix_powers")
     return V0, V1
+
+
+def warp_img2d(src, offx, offy, out=None):
+    """a17: src bilinearly warped by the offset field (offx, offy), the
+    reference's warp_img2d(src, offx, offy)."""
+    for n, t in (("src", src), ("offx", offx), ("offy", offy)):
+        _dev(t, n)
+    H, W = src.shape
+    if tuple(offx.shape) != (H, W) or tuple(offy.shape) != (H, W):
+        raise HindemithError("warp_img2d: offsets must have the image's shape")
+    dst = torch.empty_like(src) if out is None else _dev(out, "out")
+    check(lib().hmx_warp_img2d(_ptr(src), _ptr(offx), _ptr(offy), _ptr(dst), H, W,
+                               stream_ptr()), "warp_img2d")
+    return dst

@@ -327,6 +327,13 @@ int hmx_hs_matrix_powers(float lam, const float *Ix, const float *Iy, const floa
                          const float *r1, float *V0, float *V1, int H, int W, int steps,
                          void *stream);
 
+/* a17. dst = src bilinearly sampled at (x + offx, y + offy) where all four taps
+ * are in the image, else src (x, y); offsets split by truncation, one step down
+ * when negative.  Offsets must be finite with magnitude below 2^31.
+ * Replaces WarpImg2D (WarpImg2D.cpp:33-75). */
+int hmx_warp_img2d(const float *src, const float *offx, const float *offy, float *dst, int H,
+                   int W, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

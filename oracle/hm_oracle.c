@@ -572,3 +572,62 @@ void or_hs_powers_f64(double lam, const double *Ix, const double *Iy, const doub
             }
     }
 }
+
+/* ------------------------------------------------------------------------
+ * or_warp_img2d_f32: WarpImg2D.cpp:33-75, statement by statement in the types C
+ * gives them (double literals 1.0; each "out += term" one fma in the term's
+ * type, rounded to float).  or_warp_img2d_f64: the same split of the offset
+ * (truncate, one step down when negative) and the same bounds, in double.
+ */
+void or_warp_img2d_f32(const float *src, const float *offx, const float *offy, float *dst, int H,
+                       int W)
+{
+#pragma omp parallel for schedule(static)
+    for (int r = 0; r < H; ++r)
+        for (int c = 0; c < W; ++c) {
+            const int64_t p = (int64_t)r * W + c;
+            const float ox = offx[p], oy = offy[p];
+            int mx = (int)ox, my = (int)oy;
+            float xf = ox - (float)mx, yf = oy - (float)my;
+            if (ox < 0.0f) { --mx; xf = (float)(1.0 + (double)xf); }
+            if (oy < 0.0f) { --my; yf = (float)(1.0 + (double)yf); }
+            const int64_t sx = (int64_t)c + mx, sy = (int64_t)r + my;
+            if (sx >= 0 && sx + 1 < W && sy >= 0 && sy + 1 < H) {
+                const int64_t q = sy * W + sx;
+                const double ax = 1.0 - (double)xf, ay = 1.0 - (double)yf;
+                float out = 0.0f;
+                out = (float)fma((double)src[q] * ax, ay, (double)out);
+                const float p10 = src[q + 1] * xf;
+                out = (float)fma((double)p10, ay, (double)out);
+                out = (float)fma((double)src[q + W] * ax, (double)yf, (double)out);
+                const float p11 = src[q + W + 1] * xf;
+                out = fmaf(p11, yf, out);
+                dst[p] = out;
+            } else {
+                dst[p] = src[p];
+            }
+        }
+}
+
+void or_warp_img2d_f64(const double *src, const double *offx, const double *offy, double *dst,
+                       int H, int W)
+{
+#pragma omp parallel for schedule(static)
+    for (int r = 0; r < H; ++r)
+        for (int c = 0; c < W; ++c) {
+            const int64_t p = (int64_t)r * W + c;
+            const double ox = offx[p], oy = offy[p];
+            int64_t mx = (int64_t)ox, my = (int64_t)oy;
+            double xf = ox - (double)mx, yf = oy - (double)my;
+            if (ox < 0.0) { --mx; xf += 1.0; }
+            if (oy < 0.0) { --my; yf += 1.0; }
+            const int64_t sx = (int64_t)c + mx, sy = (int64_t)r + my;
+            if (sx >= 0 && sx + 1 < W && sy >= 0 && sy + 1 < H) {
+                const int64_t q = sy * W + sx;
+                dst[p] = src[q] * (1 - xf) * (1 - yf) + src[q + 1] * xf * (1 - yf) +
+                         src[q + W] * (1 - xf) * yf + src[q + W + 1] * xf * yf;
+            } else {
+                dst[p] = src[p];
+            }
+        }
+}

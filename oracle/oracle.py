@@ -15,7 +15,7 @@ __all__ = [
     "build_oracle", "load_oracle", "stencil2d", "stencil2d_T", "dia2d", "csr_spmv",
     "sum2d", "hs_setup", "hs_jacobi", "hs_solve", "hs_grad", "num_threads",
     "set_num_threads", "np_stencil2d", "np_dia2d_csr", "np_hs_solve_f64",
-    "endpoint_error", "hs_precondition", "hs_matrix_powers",
+    "endpoint_error", "hs_precondition", "hs_matrix_powers", "warp_img2d",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -290,6 +290,20 @@ def hs_matrix_powers(lam, Ix, Iy, r0, r1, steps, dtype=np.float32):
         lib.or_hs_powers_f64(ctypes.c_double(lam), _p(ix), _p(iy), _p(a), _p(b), _p(V0),
                              _p(V1), H, W, int(steps))
     return V0, V1
+
+
+def warp_img2d(src, offx, offy, dtype=np.float32):
+    """WarpImg2D.cpp:33-75: bilinear sample at (x + offx, y + offy) if all four taps
+    are inside, else src(x, y).  The offset's integer part is truncated and moved
+    down by one when negative, in both forms."""
+    lib = load_oracle()
+    cast = _f32 if dtype == np.float32 else _f64
+    s, ox, oy = cast(src), cast(offx), cast(offy)
+    H, W = s.shape
+    d = np.empty_like(s)
+    fn = lib.or_warp_img2d_f32 if dtype == np.float32 else lib.or_warp_img2d_f64
+    fn(_p(s), _p(ox), _p(oy), _p(d), H, W)
+    return d
 
 
 def endpoint_error(u, v, fx, fy, margin=8):

@@ -192,12 +192,12 @@ def test_loop_bound_must_be_literal():
 
 def test_out_of_scope_operation():
     @hm
-    def warp(x, u, v):
-        y = warp_img2d(x, u, v)  # noqa: F821
+    def lk(x, u, v):
+        y = lk_least_squares(x, u, v)  # noqa: F821
         return y
     z = np.zeros((4, 4), np.float32)
-    with pytest.raises(HindemithError, match="WarpImg2D"):
-        warp.compile(x=z, u=z, v=z)
+    with pytest.raises(HindemithError, match="LKLeastSquares"):
+        lk.compile(x=z, u=z, v=z)
 
 
 def test_unknown_symbol():

@@ -89,3 +89,27 @@ def test_numpy_twins_follow_the_oracle():
     R0, R1 = oracle.hs_matrix_powers(0.5, k["Ix"], k["Iy"], k["r0"], k["r1"], 4, np.float64)
     assert np.allclose(V0, R0, rtol=1e-12, atol=1e-12) and np.allclose(V1, R1, rtol=1e-12,
                                                                          atol=1e-12)
+
+
+@hm
+def warp_residual(I1, I2, u, v):
+    I2w = warp_img2d(I2, u, v)  # noqa: F821 (resolved by the @hm front end)
+    r = I2w - I1
+    return r, I2w
+
+
+def test_warp_program_plans_a_warp_node():
+    z = np.zeros((12, 20), np.float32)
+    plan = warp_residual.compile(I1=z, I2=z, u=z, v=z)
+    assert "warp_img2d " in plan and "at +(u, v)" in plan
+
+
+def test_warp_twin_follows_the_oracle():
+    from hindemith_amd.hindemith import warp_img2d
+    rng = np.random.default_rng(3)
+    src = rng.normal(size=(11, 17))
+    ox = rng.uniform(-3.5, 3.5, (11, 17))
+    oy = rng.uniform(-3.5, 3.5, (11, 17))
+    ox[0, :5] = [-1.0, 2.0, 0.0, -3.0, 1.0]   # whole offsets, both signs
+    assert np.allclose(warp_img2d(src, ox, oy), oracle.warp_img2d(src, ox, oy, np.float64),
+                       rtol=0, atol=1e-12)

@@ -208,6 +208,12 @@ def extra_bandwidth(K, S):
     A = torch.from_numpy(planes).cuda()
     ms = _time_graph_ms(lambda: K.dia2d_spmv(A, ox, oy, x, out=y))
     out["dia2d_5diag"] = {"us": round(ms * 1e3, 2), "GBps": round(28 * N / ms / 1e6, 1)}
+    # the same operator as a 1-D scipy-style dia_matrix of size N (offsets 0, +-1, +-W)
+    off1 = np.array([0, -1, 1, -W, W], np.int32)
+    D1 = torch.from_numpy(np.ascontiguousarray(planes.reshape(5, N))).cuda()
+    ms = _time_graph_ms(lambda: K.dia_spmv(D1, off1, x.reshape(-1), out=y.reshape(-1)))
+    out["dia1d_5diag"] = {"us": round(ms * 1e3, 2), "GBps": round(28 * N / ms / 1e6, 1)}
+    del D1
     ip, ix, d = S.dia2d_to_csr(planes, ox, oy)
     tip, tix, td = (torch.from_numpy(a).cuda() for a in (ip, ix, d))
     xf = x.reshape(-1)

@@ -91,6 +91,11 @@ def _bind_symbol(rid, name, tag, attrs, obj, keep):
         _bind(rid, name, 0, np.ascontiguousarray(obj.data, dtype=np.float32), keep)
         _bind(rid, name, 1, np.ascontiguousarray(obj.offx, dtype=np.int32), keep)
         _bind(rid, name, 2, np.ascontiguousarray(obj.offy, dtype=np.int32), keep)
+    elif tag == "SpMat":  # scipy dia_matrix: data (ndiags x n), offsets
+        if obj.dtype != np.float32:
+            raise HindemithError(f"kwarg {name}: dia_matrix data must be float32")
+        _bind(rid, name, 0, np.ascontiguousarray(obj.data), keep)
+        _bind(rid, name, 1, np.ascontiguousarray(obj.offsets, dtype=np.int32), keep)
     elif tag == "CSRMat":
         if obj.dtype != np.float32:
             raise HindemithError(f"kwarg {name}: csr_matrix data must be float32")

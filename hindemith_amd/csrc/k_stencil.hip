@@ -228,6 +228,23 @@ extern "C" int hmx_dia2d_spmv(const float *A, int nd, const int *offx, const int
     return launch_dia2d(A, x, y, H, W, t, as_stream(stream));
 }
 
+extern "C" int hmx_dia_spmv(const float *A, int nd, const int *offsets, const float *x,
+                            float *y, int n, void *stream) {
+    HMX_REQUIRE(A && x && y && offsets, "hmx_dia_spmv: null argument");
+    HMX_REQUIRE(x != y, "hmx_dia_spmv: x and y must be distinct buffers");
+    HMX_REQUIRE(n > 0, "hmx_dia_spmv: n must be positive");
+    HMX_REQUIRE(nd >= 1 && nd <= HMX_MAX_TAPS,
+                "hmx_dia_spmv: 1.." + std::to_string(HMX_MAX_TAPS) + " diagonals");
+    // a 1 x n grid with no vertical offsets: the 2D DIA rule (coefficient at
+    // the neighbour index, out-of-range terms skipped) is the 1-D one
+    static const float unused_w[HMX_MAX_TAPS] = {0};
+    int zeros[HMX_MAX_TAPS] = {0};
+    Taps t;
+    int rc = make_taps(t, nd, unused_w, offsets, zeros);
+    if (rc) return rc;
+    return launch_dia2d(A, x, y, 1, n, t, as_stream(stream));
+}
+
 extern "C" int hmx_hs_setup(const float *I1, const float *I2, int H, int W, int ndx,
                             const float *wx, const int *oxx, const int *oyx, int ndy,
                             const float *wy, const int *oxy, const int *oyy, int ndt,

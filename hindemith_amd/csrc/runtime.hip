@@ -371,6 +371,17 @@ static int parse_symtab(Record &R, const XNode &root) {
             s.ndiags = attr_i(n, "ndiags");
             s.dim0 = attr_i(n, "dim0");
             s.dim1 = attr_i(n, "dim1");
+        } else if (n.tag == "SpMat") {
+            // scipy dia_matrix, square (PyAST.py:25-26, SpMat.cpp:147-159)
+            s.kind = HMX_KIND_SPMAT;
+            s.ndiags = attr_i(n, "ndiags");
+            s.dim0 = attr_i(n, "dim");
+            s.H = 1;
+            s.W = (int)s.dim0;
+            if (s.ndiags < 1 || s.ndiags > HMX_MAX_TAPS || s.dim0 < 1 || s.dim0 > 1 << 30)
+                return fail(HMX_ERR_UNSUPPORTED, "dia_matrix " + s.name + " must have 1.." +
+                                                     std::to_string(HMX_MAX_TAPS) +
+                                                     " diagonals");
         } else if (n.tag == "CSRMat") {
             s.kind = HMX_KIND_CSRMAT;
             s.nnz = attr_i(n, "nnz");
@@ -406,7 +417,7 @@ static const std::set<std::string> kFilters = {
     "Array2DSum", "Stencil2DMVM", "SpmvDIA2D", "SpmvCSR", "ArrayOp", "ArrayScalarOp",
     "ScalarArrayOp", "ArrayCopy", "ArraySqrt", "ArraySum", "ScalarOp", "ScalarCopy",
     "ScalarSqrt", "Return", "ApplyHSPreconditioner", "HSMatrixPowers", "Array3DCopy",
-    "WarpImg2D"};
+    "WarpImg2D", "SpmvDIA"};
 
 static int parse_block(Record &R, const XNode &paf, std::vector<Stmt> &out, int &counter) {
     for (const XNode &n : paf.kids) {
@@ -520,6 +531,13 @@ static int check_stmt(Record &R, const Stmt &st) {
         const Sym &A = R.syms[st.args[1]], &x = R.syms[st.args[2]];
         if (A.dim0 != x.dim0 || A.dim1 != x.dim1)
             return fail(HMX_ERR_INVALID, "SpmvDIA2D: matrix planes and vector differ in shape");
+    } else if (f == "SpmvDIA") {
+        if ((rc = need(3))) return rc;
+        if (K(1) != HMX_KIND_SPMAT || K(2) != HMX_KIND_ARRAY || K(0) != HMX_KIND_ARRAY)
+            return fail(HMX_ERR_INVALID, "SpmvDIA expects (Array <- SpMat, Array)");
+        const Sym &A = R.syms[st.args[1]], &x = R.syms[st.args[2]];
+        if (A.dim0 != x.dim0)
+            return fail(HMX_ERR_INVALID, "SpmvDIA: matrix and vector sizes do not match");
     } else if (f == "SpmvCSR") {
         if ((rc = need(3))) return rc;
         if (K(1) != HMX_KIND_CSRMAT || K(2) != HMX_KIND_ARRAY || K(0) != HMX_KIND_ARRAY)
@@ -807,7 +825,7 @@ class Planner {
         int H = is_arr(dst) ? dst.H : src.H, W = is_arr(dst) ? dst.W : src.W;
         // read operands of the statement
         std::vector<int> rd(st.args.begin() + 1, st.args.end());
-        const bool nb = f == "Stencil2DMVM" || f == "SpmvDIA2D";
+        const bool nb = f == "Stencil2DMVM" || f == "SpmvDIA2D" || f == "SpmvDIA";
         for (int a : rd)
             if (is_arr(R_.syms[a]) && (R_.syms[a].H != H || R_.syms[a].W != W))
                 return fail(HMX_ERR_INVALID, f + ": operand " + R_.syms[a].name +
@@ -1056,6 +1074,10 @@ static void size_symbols(Record &R) {
                 s.bytes[0] = s.ndiags * s.dim0 * s.dim1 * 4;
                 s.bytes[1] = s.bytes[2] = s.ndiags * 4;
                 break;
+            case HMX_KIND_SPMAT:  // data, offsets, and zero vertical offsets (internal)
+                s.bytes[0] = s.ndiags * s.dim0 * 4;
+                s.bytes[1] = s.bytes[2] = s.ndiags * 4;
+                break;
             case HMX_KIND_CSRMAT:
                 s.bytes[0] = s.bytes[1] = s.nnz * 4;
                 s.bytes[2] = (s.dim0 + 1) * 4;
@@ -1075,6 +1097,12 @@ static int allocate(Record &R) {
             case HMX_KIND_SCALAR: rc = dalloc(R, &s.dev[0], 8); break;
             case HMX_KIND_ARRAY3D:
                 if (s.needs_buffer) rc = dalloc(R, &s.dev[0], s.bytes[0]);
+                break;
+            case HMX_KIND_SPMAT:
+                for (int k = 0; k < 3 && !rc; ++k) rc = dalloc(R, &s.dev[k], s.bytes[k]);
+                // the vertical offsets of the 1 x n grid are all zero
+                if (!rc && hipMemset(s.dev[2], 0, (size_t)s.bytes[2]) != hipSuccess)
+                    rc = fail(HMX_ERR_HIP, "clearing the offsets of " + s.name);
                 break;
             default:
                 for (int k = 0; k < 3 && !rc; ++k) rc = dalloc(R, &s.dev[k], s.bytes[k]);
@@ -1120,7 +1148,7 @@ static int run_group(Record &R, Group &g) {
         a.opnd[i].ox = (const int *)s.dev[1];
         a.opnd[i].oy = (const int *)s.dev[2];
         a.opnd[i].nd = (int)s.ndiags;
-        a.opnd[i].kind = s.kind == HMX_KIND_SPMAT2D;
+        a.opnd[i].kind = s.kind == HMX_KIND_SPMAT2D || s.kind == HMX_KIND_SPMAT;
     }
     for (size_t k = 0; k < g.red_syms.size(); ++k) a.red_part[k] = g.partials + k * g.nblocks;
     int rc = launch_program(a, R.stream, nullptr);
@@ -1489,7 +1517,8 @@ extern "C" int hmx_bind(int rec, const char *name, int component, const void *ho
     if (!s.header) return fail(HMX_ERR_INVALID, s.name + " is not a Header (kwarg) symbol");
     if (component < 0 || component > 2 || (component > 0 && (s.kind == HMX_KIND_ARRAY ||
                                                             s.kind == HMX_KIND_ARRAY2D ||
-                                                            s.kind == HMX_KIND_SCALAR)))
+                                                            s.kind == HMX_KIND_SCALAR)) ||
+        (component > 1 && s.kind == HMX_KIND_SPMAT))
         return fail(HMX_ERR_INVALID, s.name + ": bad component " + std::to_string(component));
     if (nbytes != s.bytes[component])
         return fail(HMX_ERR_INVALID, s.name + "[" + std::to_string(component) + "]: expected " +
@@ -1512,7 +1541,7 @@ extern "C" int hmx_run(int rec) {
     for (Sym &s : R->syms) {
         if (!s.header) continue;
         int ncomp = (s.kind == HMX_KIND_STENCIL || s.kind == HMX_KIND_SPMAT2D ||
-                     s.kind == HMX_KIND_CSRMAT) ? 3 : 1;
+                     s.kind == HMX_KIND_CSRMAT) ? 3 : (s.kind == HMX_KIND_SPMAT ? 2 : 1);
         for (int k = 0; k < ncomp; ++k) {
             if (s.bytes[k] == 0) continue;
             if (!s.host[k])

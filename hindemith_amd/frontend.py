@@ -16,6 +16,7 @@ Python syntax          operand kinds                                Filter (Temp
 ``S * x``              Stencil, Array2D                             Stencil2DMVM (:601-617)
 ``A * x``              SpMat2D, Array2D                             SpmvDIA2D (:583-599)
 ``A * x``              csr_matrix, Array                            SpmvCSR (:619-636)
+``A * x``              dia_matrix, Array                            SpmvDIA (:565-581)
 ``sum2d(a)``           Array2D                                      Array2DSum (:463-478)
 ``sum(a)``             Array                                        ArraySum (:446-461)
 ``sqrt(x)``            Scalar / Array / Array2D                     ScalarSqrt, ArraySqrt,
@@ -132,6 +133,10 @@ class CSRMat(Node):
     pass
 
 
+class SpMat(Node):
+    pass
+
+
 class Array(Node):
     pass
 
@@ -175,7 +180,7 @@ def defines_xml(defines):
 # ---------------------------------------------------------------------------
 # Prototype: kwargs -> Header symbols (PyAST.py:11-41)
 
-_OUT_OF_SCOPE_TYPES = {"dia_matrix": "SpMat (1-D DIA)", "complex64": "complex scalars"}
+_OUT_OF_SCOPE_TYPES = {"complex64": "complex scalars"}
 
 
 def _make_header(symtab, name, obj):
@@ -187,6 +192,13 @@ def _make_header(symtab, name, obj):
     elif cls == "Stencil":
         symtab[name] = Stencil(name, ndiags=len(obj.data), source="Header", io="inout",
                                dtype="float32")
+    elif cls == "dia_matrix":  # PyAST.py:25-26
+        n0, n1 = obj.shape
+        if n0 != n1 or obj.data.shape[1] != n1:
+            raise HindemithError(f"kwarg {name}: dia_matrix must be square with data of shape "
+                                 "(ndiags, n) (SpMat.cpp:147-152 indexes it that way)")
+        symtab[name] = SpMat(name, shape=tuple(obj.shape), source="Header", io="inout",
+                             dtype=str(obj.dtype), ndiags=int(obj.data.shape[0]), dim=n0)
     elif cls in ("csr_matrix", "csr_array"):
         symtab[name] = CSRMat(name, shape=tuple(obj.shape), source="Header", io="inout",
                               dtype=str(obj.dtype), nnz=int(obj.nnz), dim=obj.shape[0],
@@ -320,10 +332,12 @@ class Emitter:
         lk, rk = self.kind(lv, where), self.kind(rv, where)
         op = type(n.op).__name__
         pair = (lk, rk)
-        if pair in (("Stencil", "Array2D"), ("SpMat2D", "Array2D"), ("CSRMat", "Array")):
+        if pair in (("Stencil", "Array2D"), ("SpMat2D", "Array2D"), ("CSRMat", "Array"),
+                    ("SpMat", "Array")):
             if op != "Mult":
                 raise HindemithError(f"{where}: only '*' applies a {lk} to an array")
-            fname = {"Stencil": "Stencil2DMVM", "SpMat2D": "SpmvDIA2D", "CSRMat": "SpmvCSR"}[lk]
+            fname = {"Stencil": "Stencil2DMVM", "SpMat2D": "SpmvDIA2D", "CSRMat": "SpmvCSR",
+                     "SpMat": "SpmvDIA"}[lk]
             m = Filter(fname)
             if lk == "CSRMat":
                 rows = int(self.symtab[lv.properties["name"]].properties["dim0"])
@@ -332,6 +346,8 @@ class Emitter:
                                         dtype=self.symtab[rv.properties["name"]].properties["dtype"],
                                         dim0=rows, dim1=0, shape=(rows,))
                 newvar = Name(nm)
+            elif lk == "SpMat":  # Templates.py:574: an Array shaped like x
+                newvar = self.create_copy(Array, _unique_name(), rv)
             else:
                 newvar = self.create_copy(Array2D, _unique_name(), rv)
         else:
@@ -596,6 +612,8 @@ def _kw_signature(kwargs):
             ent += [len(v.data)]
         elif cls == "SpMat2D":
             ent += [v.data.shape]
+        elif cls == "dia_matrix":
+            ent += [v.shape, v.data.shape, str(v.dtype)]
         elif cls in ("csr_matrix", "csr_array"):
             ent += [v.shape, int(v.nnz), str(v.dtype)]
         sig.append(tuple(ent))

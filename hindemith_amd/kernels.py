@@ -19,7 +19,7 @@ from ._lib import HindemithError, check, lib
 __all__ = [
     "taps_of", "stencil2d", "dia2d_spmv", "csr_spmv", "CSRBins", "csr_spmv_binned",
     "sum2d", "hs_setup", "hs_jacobi", "hs_solve", "hs_tune", "hs_variants", "stream_ptr",
-    "hs_precondition", "hs_matrix_powers", "warp_img2d",
+    "hs_precondition", "hs_matrix_powers", "warp_img2d", "dia_spmv",
 ]
 
 
@@ -276,3 +276,20 @@ def warp_img2d(src, offx, offy, out=None):
     check(lib().hmx_warp_img2d(_ptr(src), _ptr(offx), _ptr(offy), _ptr(dst), H, W,
                                stream_ptr()), "warp_img2d")
     return dst
+
+
+def dia_spmv(data, offsets, x, out=None):
+    """1-D DIA SpMV of a square scipy-style dia_matrix: data (ndiags x n) on the
+    device, offsets on the host; y[i] = sum_d data[d][i+o_d] * x[i+o_d]."""
+    _dev(data, "data")
+    _dev(x, "x")
+    nd, n = data.shape
+    if x.numel() != n:
+        raise HindemithError("dia_spmv: vector length differs from the matrix size")
+    off = np.ascontiguousarray(offsets, np.int32)
+    if len(off) != nd:
+        raise HindemithError("dia_spmv: one offset per diagonal")
+    y = torch.empty_like(x) if out is None else _dev(out, "out")
+    check(lib().hmx_dia_spmv(_ptr(data), nd, off.ctypes.data_as(ctypes.c_void_p), _ptr(x), _ptr(y),
+                             n, stream_ptr()), "dia_spmv")
+    return y

@@ -48,7 +48,8 @@ enum hmx_kind {
     HMX_KIND_STENCIL = 4,
     HMX_KIND_SPMAT2D = 5,
     HMX_KIND_CSRMAT = 6,
-    HMX_KIND_ARRAY3D = 7 /* only as the HSMatrixPowers result (dim0 = STEPS planes) */
+    HMX_KIND_ARRAY3D = 7, /* only as the HSMatrixPowers result (dim0 = STEPS planes) */
+    HMX_KIND_SPMAT = 8    /* 1-D DIA matrix (scipy dia_matrix): data, offsets */
 };
 
 const char *hmx_last_error(void);
@@ -326,6 +327,12 @@ int hmx_hs_precondition(const float *diag0, const float *diag1, const float *off
 int hmx_hs_matrix_powers(float lam, const float *Ix, const float *Iy, const float *r0,
                          const float *r1, float *V0, float *V1, int H, int W, int steps,
                          void *stream);
+
+/* 1-D DIA SpMV (scipy dia_matrix, square n x n, data nd x n):
+ * y[i] = sum_d [0 <= i+off_d < n] A[d][i+off_d] * x[i+off_d], fma'd in
+ * diagonal order from 0.  Replaces SpmvDIA (SpmvDIA.cpp:30-50). */
+int hmx_dia_spmv(const float *A, int nd, const int *offsets, const float *x, float *y, int n,
+                 void *stream);
 
 /* a17. dst = src bilinearly sampled at (x + offx, y + offy) where all four taps
  * are in the image, else src (x, y); offsets split by truncation, one step down

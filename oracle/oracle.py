@@ -15,7 +15,7 @@ __all__ = [
     "build_oracle", "load_oracle", "stencil2d", "stencil2d_T", "dia2d", "csr_spmv",
     "sum2d", "hs_setup", "hs_jacobi", "hs_solve", "hs_grad", "num_threads",
     "set_num_threads", "np_stencil2d", "np_dia2d_csr", "np_hs_solve_f64",
-    "endpoint_error", "hs_precondition", "hs_matrix_powers", "warp_img2d",
+    "endpoint_error", "hs_precondition", "hs_matrix_powers", "warp_img2d", "dia1d",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -109,6 +109,17 @@ def dia2d(A, offx, offy, x, dtype=np.float32):
     fn = lib.or_dia2d_f32 if dtype == np.float32 else lib.or_dia2d_f64
     fn(_p(A), nd, _p(ox), _p(oy), _p(x), H, W, _p(y))
     return y
+
+
+def dia1d(data, offsets, x, dtype=np.float32):
+    """SpmvDIA.cpp:30-50 (1-D DIA, square, data ndiags x n): the 2D DIA rule on a
+    1 x n grid with zero vertical offsets (coefficient at i+o, out-of-range terms
+    skipped, fma in diagonal order)."""
+    data = np.asarray(data)
+    nd, n = data.shape
+    y = dia2d(data.reshape(nd, 1, n), offsets, np.zeros(nd, np.int32),
+              np.asarray(x).reshape(1, n), dtype)
+    return y.reshape(n)
 
 
 def csr_spmv(indptr, indices, data, x, dtype=np.float32):

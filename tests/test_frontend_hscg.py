@@ -113,3 +113,34 @@ def test_warp_twin_follows_the_oracle():
     ox[0, :5] = [-1.0, 2.0, 0.0, -3.0, 1.0]   # whole offsets, both signs
     assert np.allclose(warp_img2d(src, ox, oy), oracle.warp_img2d(src, ox, oy, np.float64),
                        rtol=0, atol=1e-12)
+
+
+def test_dia_matrix_program_plans_into_a_fused_program():
+    import scipy.sparse as sp
+    defines.update(nd1=1, nt1=128, ne1=2)
+
+    @hm
+    def residual(A, x, b):
+        r = b - A * x
+        s = sum(r * r)
+        return r, s
+    n = 64
+    A = sp.dia_matrix((np.ones((3, n), np.float32), np.array([0, -1, 1])), shape=(n, n))
+    z = np.zeros(n, np.float32)
+    plan = residual.compile(A=A, x=z, b=z)
+    assert "1 stencil/dia taps" in plan and "reduce 1" in plan
+    rec = fe.mycompile(residual.source, dict(A=A, x=z, b=z), defines, fn_key=("dia", 1))
+    assert 'Filter name="SpmvDIA"' in rec.model_xml and "<SpMat " in rec.symtab_xml
+
+
+def test_dia_matrix_must_be_square():
+    import scipy.sparse as sp
+    defines.update(nd1=1, nt1=128, ne1=2)
+
+    @hm
+    def f(A, x):
+        y = A * x
+        return y
+    A = sp.dia_matrix((np.ones((1, 6), np.float32), np.array([0])), shape=(4, 6))
+    with pytest.raises(HindemithError, match="square"):
+        f.compile(A=A, x=np.zeros(6, np.float32))

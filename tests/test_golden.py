@@ -51,3 +51,15 @@ def test_hs_fixture():
     assert np.allclose(u, G["hs_u"], atol=1e-12) and np.allclose(v, G["hs_v"], atol=1e-12)
     u32, v32 = oracle.hs_solve(G["hs_I1"], G["hs_I2"], 1.0, TAPS, int(G["hs_iters"]))
     assert np.max(np.abs(u32 - G["hs_u"])) < 1e-5 and np.max(np.abs(v32 - G["hs_v"])) < 1e-5
+
+
+@pytest.mark.parametrize("n", [1, 7, 40])
+def test_dia1d_matches_scipy_dia_matrix(n):
+    """1-D DIA (SpmvDIA.cpp:30-50) against scipy's own dia_matrix product."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(n)
+    offsets = np.array([0, -1, 1, -3, 5], np.int32)
+    data = rng.normal(size=(5, n))
+    x = rng.normal(size=n)
+    A = sp.dia_matrix((data, offsets), shape=(n, n))
+    assert np.allclose(oracle.dia1d(data, offsets, x, np.float64), A @ x, rtol=0, atol=1e-13)

@@ -194,3 +194,25 @@ def test_stencil_program_bitexact_wide(env):
     for _ in range(3):
         ref = oracle.stencil2d(ref, st)
     assert np.array_equal(y, ref)
+
+
+def test_dia_matrix_residual_program():
+    """scipy dia_matrix * Array through the @hm path (1-D DIA fused into the
+    elementwise program) against the oracle."""
+    from hindemith_amd.hindemith import defines, hm
+    defines.update(nd1=1, nt1=128, ne1=2)
+
+    @hm
+    def residual(A, x, b):
+        r = b - A * x
+        return r
+    rng = np.random.default_rng(1)
+    n = 5000
+    offsets = np.array([0, -1, 1, -70, 70], np.int32)
+    data = rng.normal(size=(5, n)).astype(np.float32)
+    A = sp.dia_matrix((data, offsets), shape=(n, n))
+    x = rng.normal(size=n).astype(np.float32)
+    b = rng.normal(size=n).astype(np.float32)
+    r = residual(A=A, x=x, b=b)
+    ref = (b - oracle.dia1d(data, offsets, x)).astype(np.float32)
+    assert np.array_equal(r, ref)

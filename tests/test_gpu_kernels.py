@@ -231,3 +231,14 @@ def test_hs_jacobi_tap_classes_bitexact(K, kind):
         v = torch.zeros(shape, device="cuda")
         K.hs_jacobi(dev(Ix), dev(Iy), dev(It), 0.8, avg, 11, u, v, variant=vi, k=k)
         assert np.array_equal(host(u), ref_u) and np.array_equal(host(v), ref_v), (kind, vi, k)
+
+
+@pytest.mark.parametrize("n", [1, 33, 4097])
+def test_dia1d_bitexact(K, n):
+    """1-D DIA SpMV (scipy dia_matrix layout) against the fp32 oracle."""
+    rng = np.random.default_rng(n)
+    offsets = np.array([0, 1, -1, 7, -64], np.int32)
+    data = rng.normal(size=(5, n)).astype(np.float32)
+    x = rng.normal(size=n).astype(np.float32)
+    y = K.dia_spmv(torch.from_numpy(data).cuda(), offsets, torch.from_numpy(x).cuda())
+    assert np.array_equal(y.cpu().numpy(), oracle.dia1d(data, offsets, x))

@@ -269,8 +269,9 @@ def extra_batched(S):
 
 
 def extra_hscg(K, S):
-    """The preconditioned-CG operations and the image warp at 1920x1080 fp32,
-    timed as hipGraph replays: the warp of frame 2 by the synthetic flow (16 B/px:
+    """The preconditioned-CG operations, the Brox SOR blocks and the image warp at
+    1920x1080 fp32, timed as hipGraph replays: the warp of frame 2 by the synthetic
+    flow (16 B/px:
     read src, offx, offy; write dst), the 2x2 block-Jacobi preconditioner (28 B/px: read diag0, diag1,
     offdiag, x0, x1; write y0, y1) and 8 HS operator powers (algorithmic bytes
     as the reference's one launch per power: V[0] copy 16 B/px + 24 B/px per
@@ -297,8 +298,20 @@ def extra_hscg(K, S):
     wdst = torch.empty_like(r0)
     I2d = torch.from_numpy(I2).cuda()
     ms_w = _time_graph_ms(lambda: K.warp_img2d(I2d, fx, fy, out=wdst))
+    # Brox SOR blocks: the psi-weighted Laplacian into 5 planes (24 B/px: read psi,
+    # write 5 planes) and the 2x2 preconditioner on its diagonal (32 B/px)
+    Ab = torch.zeros((5, H, W), device="cuda")
+    psi = (Ix * Ix + 1.0).contiguous()
+    ms_bs = _time_graph_ms(lambda: K.brox_set_matrix(Ab, psi, 0.5))
+    yb = (torch.empty_like(r0), torch.empty_like(r0))
+    ms_bp = _time_graph_ms(lambda: K.brox_precondition(Ab, d0, d1, od, r0, r1, out=yb))
+    del Ab
     return {"workload": f"{W}x{H} fp32",
             "warp_img2d": {"us": round(ms_w * 1e3, 2), "GBps": round(16 * n / ms_w / 1e6, 1)},
+            "brox_set_matrix": {"us": round(ms_bs * 1e3, 2),
+                                "GBps": round(24 * n / ms_bs / 1e6, 1)},
+            "brox_precondition": {"us": round(ms_bp * 1e3, 2),
+                                  "GBps": round(32 * n / ms_bp / 1e6, 1)},
             "precondition": {"us": round(ms_p * 1e3, 2), "GBps": round(28 * n / ms_p / 1e6, 1)},
             "matrix_powers_8": {"us": round(ms_m * 1e3, 2),
                                 "Mpx_power_per_s": round(n * (steps - 1) / ms_m / 1e3, 1),

@@ -67,6 +67,9 @@ _SIGS = {
     "hmx_hs_matrix_powers": [c_float, vp, vp, vp, vp, vp, vp, c_int, c_int, c_int, vp],
     "hmx_warp_img2d": [vp, vp, vp, vp, c_int, c_int, vp],
     "hmx_dia_spmv": [vp, c_int, vp, vp, vp, c_int, vp],
+    "hmx_brox_set_matrix": [vp, c_int, vp, c_float, c_int, c_int, vp],
+    "hmx_brox_set_redblack_matrix": [vp, vp, c_int, vp, c_float, c_int, c_int, vp],
+    "hmx_brox_precondition": [vp, c_int, vp, vp, vp, vp, vp, vp, vp, c_int, c_int, vp],
     "hmx_hs_batched_release": [],
     "hmx_hs_ckpt_count": [c_int, c_int],
     "hmx_hs_forward_ckpt": [vp, vp, c_int, c_int, c_int, vp] + [c_int, vp, vp, vp] * 4 +

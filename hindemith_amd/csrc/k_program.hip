@@ -67,6 +67,8 @@ __global__ __launch_bounds__(PROG_NT) void program_kernel(ProgArgs a) {
                     case HMX_OP_MUL: v = reg[in.a][tid] * reg[in.b][tid]; break;
                     case HMX_OP_DIV: v = reg[in.a][tid] / reg[in.b][tid]; break;
                     case HMX_OP_SQRT: v = sqrtf(reg[in.a][tid]); break;
+                    case HMX_OP_RED: v = ((c + r) & 1) ? 0.0f : reg[in.a][tid]; break;
+                    case HMX_OP_BLACK: v = ((c + r) & 1) ? reg[in.a][tid] : 0.0f; break;
                     case HMX_OP_STORE: a.out[in.dst][e] = reg[in.a][tid]; continue;
                     case HMX_OP_REDUCE:
                         if (in.dst == 0) acc[0] += (double)reg[in.a][tid];

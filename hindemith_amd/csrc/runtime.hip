@@ -49,6 +49,11 @@ int launch_hs_powers(const float *Ix, const float *Iy, const float *r0, const fl
                      int lam_dtype, float lam_val, hipStream_t s);
 int launch_warp_img2d(const float *src, const float *offx, const float *offy, float *dst, int H,
                       int W, hipStream_t s);
+int launch_brox_set(float *A, float *diag, const float *psi, int H, int W, const void *alpha_dev,
+                    int alpha_dt, float alpha_val, int rb, hipStream_t s);
+int launch_brox_precond(const float *A2, const float *diag0, const float *diag1,
+                        const float *offdiag, const float *x0, const float *x1, float *y0,
+                        float *y1, int H, int W, hipStream_t s);
 
 namespace rt {
 
@@ -274,12 +279,16 @@ struct Group {
 };
 
 struct Node {
-    enum T { GROUP, SCALAR, CSR, LOOP, HS, RETURN, PRECOND, POWERS, COPY3D, WARP } t;
+    enum T {
+        GROUP, SCALAR, CSR, LOOP, HS, RETURN, PRECOND, POWERS, COPY3D, WARP, BROX_SET,
+        BROX_SET_RB, BROX_PRECOND
+    } t;
     int idx = 0;                 // GROUP: groups[], SCALAR: sprogs[]
     int ub = 0;                  // LOOP / HS
     std::vector<Node> body;      // LOOP (and HS fallback)
     std::vector<int> syms;       // CSR: y, A, x ; HS: u, v, Ix, Iy, It, den, Avg ; RETURN ;
-                                 // PRECOND/POWERS/COPY3D/WARP: the statement's operands
+                                 // PRECOND/POWERS/COPY3D/WARP/BROX_*: the statement's
+                                 // operands
 };
 
 struct ScalarStmt {
@@ -417,7 +426,8 @@ static const std::set<std::string> kFilters = {
     "Array2DSum", "Stencil2DMVM", "SpmvDIA2D", "SpmvCSR", "ArrayOp", "ArrayScalarOp",
     "ScalarArrayOp", "ArrayCopy", "ArraySqrt", "ArraySum", "ScalarOp", "ScalarCopy",
     "ScalarSqrt", "Return", "ApplyHSPreconditioner", "HSMatrixPowers", "Array3DCopy",
-    "WarpImg2D", "SpmvDIA"};
+    "WarpImg2D", "SpmvDIA", "SetBroxMatrix", "SetBroxRedblackMatrix",
+    "ApplyBroxPreconditioner", "RedArray2D", "BlackArray2D"};
 
 static int parse_block(Record &R, const XNode &paf, std::vector<Stmt> &out, int &counter) {
     for (const XNode &n : paf.kids) {
@@ -586,6 +596,43 @@ static int check_stmt(Record &R, const Stmt &st) {
         if (std::strtoll(it->second.c_str(), nullptr, 10) != V.dim0)
             return fail(HMX_ERR_INVALID, "HSMatrixPowers: result has " + std::to_string(V.dim0) +
                                              " planes, defines['STEPS'] is " + it->second);
+    } else if (f == "SetBroxMatrix" || f == "SetBroxRedblackMatrix") {
+        // (A, psi, alpha) / (Adiag <- A, psi, alpha), Templates.py:311-350
+        const bool rb = f == "SetBroxRedblackMatrix";
+        if ((rc = need(rb ? 4 : 3))) return rc;
+        const int o = rb ? 1 : 0;
+        const Sym &A = R.syms[st.args[o]], &psi = R.syms[st.args[o + 1]];
+        if (A.kind != HMX_KIND_SPMAT2D || psi.kind != HMX_KIND_ARRAY2D ||
+            R.syms[st.args[o + 2]].kind != HMX_KIND_SCALAR)
+            return fail(HMX_ERR_INVALID, f + " expects (SpMat2D, Array2D, Scalar)");
+        if (A.ndiags < (rb ? 4 : 5))
+            return fail(HMX_ERR_INVALID, f + ": " + A.name + " needs " +
+                                             std::to_string(rb ? 4 : 5) + " diagonal planes");
+        if (A.dim0 * A.dim1 != (int64_t)psi.H * psi.W || psi.col_mjr)
+            return fail(HMX_ERR_INVALID, f + ": " + A.name + " and " + psi.name +
+                                             " differ in shape (or psi is column-major)");
+        if (rb && (R.syms[st.args[0]].H != psi.H || R.syms[st.args[0]].W != psi.W))
+            return fail(HMX_ERR_INVALID, f + ": the diagonal must be shaped like psi");
+    } else if (f == "ApplyBroxPreconditioner") {
+        // (y0, y1 <- A, diag0, diag1, offdiag, x0, x1), Templates.py:375-400
+        if ((rc = need(8))) return rc;
+        const Sym &A = R.syms[st.args[2]], &x = R.syms[st.args[6]];
+        if (A.kind != HMX_KIND_SPMAT2D || A.ndiags < 3)
+            return fail(HMX_ERR_INVALID, "ApplyBroxPreconditioner: the first argument must be "
+                                         "an SpMat2D with a diagonal plane 2");
+        if (A.dim0 * A.dim1 != (int64_t)x.H * x.W)
+            return fail(HMX_ERR_INVALID, "ApplyBroxPreconditioner: matrix and arrays differ");
+        for (int i : {0, 1, 3, 4, 5, 6, 7}) {
+            const Sym &a = R.syms[st.args[i]];
+            if (a.kind != HMX_KIND_ARRAY2D || a.H != x.H || a.W != x.W)
+                return fail(HMX_ERR_INVALID, "ApplyBroxPreconditioner: " + a.name +
+                                                 " must be an Array2D shaped like " + x.name);
+        }
+    } else if (f == "RedArray2D" || f == "BlackArray2D") {
+        if ((rc = need(2))) return rc;
+        if (!is_arr(R.syms[st.args[0]]) || !is_arr(R.syms[st.args[1]]) ||
+            R.syms[st.args[1]].kind != HMX_KIND_ARRAY2D)
+            return fail(HMX_ERR_INVALID, f + " operands must be Array2Ds");
     } else if (f == "WarpImg2D") {
         // (dst <- src, offx, offy), Templates.py:257-274
         if ((rc = need(4))) return rc;
@@ -625,7 +672,8 @@ static void collect_reads(Record &R, const std::vector<Stmt> &b) {
         size_t first = st.filter == "Return"
                            ? 0
                            : ((st.filter == "ApplyHSPreconditioner" ||
-                               st.filter == "HSMatrixPowers") ? 2 : 1);
+                               st.filter == "HSMatrixPowers" ||
+                               st.filter == "ApplyBroxPreconditioner") ? 2 : 1);
         for (size_t i = first; i < st.args.size(); ++i) R.reads[st.args[i]].push_back(st.index);
     }
 }
@@ -696,15 +744,16 @@ class Planner {
                 out.push_back(n);
                 continue;
             }
-            if (f == "ApplyHSPreconditioner" || f == "HSMatrixPowers" || f == "Array3DCopy" ||
-                f == "WarpImg2D") {
+            static const std::map<std::string, Node::T> kNodeOf = {
+                {"ApplyHSPreconditioner", Node::PRECOND}, {"HSMatrixPowers", Node::POWERS},
+                {"Array3DCopy", Node::COPY3D},           {"WarpImg2D", Node::WARP},
+                {"SetBroxMatrix", Node::BROX_SET},       {"SetBroxRedblackMatrix", Node::BROX_SET_RB},
+                {"ApplyBroxPreconditioner", Node::BROX_PRECOND}};
+            auto kn = kNodeOf.find(f);
+            if (kn != kNodeOf.end()) {
                 if ((rc = flush(out))) return rc;
                 Node n;
-                n.t = f == "ApplyHSPreconditioner"
-                          ? Node::PRECOND
-                          : (f == "HSMatrixPowers"
-                                 ? Node::POWERS
-                                 : (f == "WarpImg2D" ? Node::WARP : Node::COPY3D));
+                n.t = kn->second;
                 n.syms = st.args;
                 out.push_back(n);
                 continue;
@@ -873,6 +922,10 @@ class Planner {
             write(g, st.args[0], reg_of(g, st.args[1]), idx);
         } else if (f == "Array2DSqrt" || f == "ArraySqrt") {
             write(g, st.args[0], emit(g, HMX_OP_SQRT, reg_of(g, st.args[1]), 0), idx);
+        } else if (f == "RedArray2D" || f == "BlackArray2D") {
+            write(g, st.args[0],
+                  emit(g, f == "RedArray2D" ? HMX_OP_RED : HMX_OP_BLACK, reg_of(g, st.args[1]), 0),
+                  idx);
         } else if (f == "Array2DSum" || f == "ArraySum") {
             int r = reg_of(g, st.args[1]);
             int slot = (int)g.red_syms.size();
@@ -897,7 +950,10 @@ class Planner {
             switch (in.op) {
                 case HMX_OP_ADD: case HMX_OP_SUB: case HMX_OP_MUL: case HMX_OP_DIV:
                     u0 = in.a; u1 = in.b; break;
-                case HMX_OP_SQRT: case HMX_OP_STORE: case HMX_OP_REDUCE: u0 = in.a; break;
+                case HMX_OP_SQRT: case HMX_OP_STORE: case HMX_OP_REDUCE: case HMX_OP_RED:
+                case HMX_OP_BLACK:
+                    u0 = in.a;
+                    break;
                 default: break;
             }
         };
@@ -1050,7 +1106,8 @@ static void mark_buffers(Record &R, const std::vector<Node> &nodes) {
                 break;
             }
             case Node::CSR: case Node::RETURN: case Node::HS: case Node::PRECOND:
-            case Node::POWERS: case Node::COPY3D: case Node::WARP:
+            case Node::POWERS: case Node::COPY3D: case Node::WARP: case Node::BROX_SET:
+            case Node::BROX_SET_RB: case Node::BROX_PRECOND:
                 for (int s : n.syms) R.syms[s].needs_buffer = true;
                 mark_buffers(R, n.body);
                 break;
@@ -1230,6 +1287,32 @@ static int exec_node(Record &R, Node &n) {
                                     (float *)V0.dev[0], (float *)V1.dev[0], r.H, r.W,
                                     (int)V0.dim0, lam.dev[0], lam.dtype, 0.0f, R.stream);
         }
+        case Node::BROX_SET: case Node::BROX_SET_RB: {
+            const bool rb = n.t == Node::BROX_SET_RB;
+            const int o = rb ? 1 : 0;
+            Sym &A = R.syms[n.syms[o]], &psi = R.syms[n.syms[o + 1]], &al = R.syms[n.syms[o + 2]];
+            float *diag = nullptr;
+            Sym *D = rb ? &R.syms[n.syms[0]] : nullptr;
+            if (D) diag = D->renamed ? D->other_buf() : D->cur_buf();
+            int rc = launch_brox_set((float *)A.dev[0], diag, psi.cur_buf(), psi.H, psi.W,
+                                     al.dev[0], al.dtype, 0.0f, rb ? 1 : 0, R.stream);
+            if (!rc && D && D->renamed) D->cur ^= 1;
+            return rc;
+        }
+        case Node::BROX_PRECOND: {
+            Sym &y0 = R.syms[n.syms[0]], &y1 = R.syms[n.syms[1]], &A = R.syms[n.syms[2]];
+            const Sym &x = R.syms[n.syms[6]];
+            int rc = launch_brox_precond((const float *)A.dev[0] + 2 * (int64_t)x.H * x.W,
+                                         R.syms[n.syms[3]].cur_buf(), R.syms[n.syms[4]].cur_buf(),
+                                         R.syms[n.syms[5]].cur_buf(), x.cur_buf(),
+                                         R.syms[n.syms[7]].cur_buf(),
+                                         y0.renamed ? y0.other_buf() : y0.cur_buf(),
+                                         y1.renamed ? y1.other_buf() : y1.cur_buf(), x.H, x.W,
+                                         R.stream);
+            if (!rc && y0.renamed) y0.cur ^= 1;
+            if (!rc && y1.renamed) y1.cur ^= 1;
+            return rc;
+        }
         case Node::WARP: {
             Sym &d = R.syms[n.syms[0]], &src = R.syms[n.syms[1]];
             int rc = launch_warp_img2d(src.cur_buf(), R.syms[n.syms[2]].cur_buf(),
@@ -1343,6 +1426,19 @@ static void describe(const Record &R, const std::vector<Node> &nodes, int depth,
                    << R.syms[n.syms[5]].name << ", " << R.syms[n.syms[6]].name
                    << "), lam=" << R.syms[n.syms[2]].name << "\n";
                 break;
+            case Node::BROX_SET:
+                os << pad << "brox_set_matrix " << R.syms[n.syms[0]].name << " from psi="
+                   << R.syms[n.syms[1]].name << "\n";
+                break;
+            case Node::BROX_SET_RB:
+                os << pad << "brox_set_redblack_matrix " << R.syms[n.syms[1]].name << ", diag "
+                   << R.syms[n.syms[0]].name << " from psi=" << R.syms[n.syms[2]].name << "\n";
+                break;
+            case Node::BROX_PRECOND:
+                os << pad << "brox_precondition (" << R.syms[n.syms[0]].name << ", "
+                   << R.syms[n.syms[1]].name << ") = M^-1 (" << R.syms[n.syms[6]].name << ", "
+                   << R.syms[n.syms[7]].name << "), A=" << R.syms[n.syms[2]].name << "\n";
+                break;
             case Node::WARP:
                 os << pad << "warp_img2d " << R.syms[n.syms[0]].name << " = "
                    << R.syms[n.syms[1]].name << " at +(" << R.syms[n.syms[2]].name << ", "
@@ -1377,7 +1473,8 @@ static int check_defines(Record &R, const std::vector<Stmt> &b, bool &need2d, bo
         // the preconditioned-CG operations launch 2D (ApplyHSPreconditioner.cpp:111-116,
         // HSMatrixPowers.cpp:147-153)
         if (st.filter.find("2D") != std::string::npos || st.filter.find("3D") != std::string::npos ||
-            st.filter == "ApplyHSPreconditioner" || st.filter == "HSMatrixPowers")
+            st.filter == "ApplyHSPreconditioner" || st.filter == "HSMatrixPowers" ||
+            st.filter.find("Brox") != std::string::npos)
             need2d = true;
         else need1d = true;
     }

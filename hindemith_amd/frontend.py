@@ -29,6 +29,10 @@ Python syntax          operand kinds                                Filter (Temp
                        Scalar, Array2D x 4 -> (Array3D, Array3D)    (:277-300), STEPS planes
 ``V = <Array3D>``      Array3D                                      Array3DCopy (:698-730)
 ``warp_img2d(src, ox, oy)`` Array2D x 3 -> Array2D                  WarpImg2D (:257-274)
+``set_brox_matrix(A, psi, alpha)`` statement, writes A's planes      SetBroxMatrix (:311-330)
+``set_brox_redblack_matrix(A, psi, alpha)`` -> diagonal Array2D      SetBroxRedblackMatrix
+``applyBroxPreconditioner(A, d0, d1, od, x0, x1)`` -> 2 Array2Ds     ApplyBroxPreconditioner
+``red(x)`` / ``black(x)``  Array2D -> Array2D                         RedArray2D, BlackArray2D
 ``for i in range(N)``  N an int literal                             Iterator (:57-71)
 ``return a, b``                                                     Return (:760-780)
 =====================  ===========================================  =====================
@@ -229,10 +233,7 @@ def _make_header(symtab, name, obj):
 
 _BINOPS = {"Add", "Sub", "Mult", "Div"}
 _OUT_OF_SCOPE_CALLS = {
-    "lk_least_squares": "LKLeastSquares",
-    "applyBroxPreconditioner": "ApplyBroxPreconditioner", "set_brox_matrix": "SetBroxMatrix",
-    "set_brox_redblack_matrix": "SetBroxRedblackMatrix", "red": "RedArray2D",
-    "black": "BlackArray2D", "extract_snapshots": "ExtractSnapshots",
+    "lk_least_squares": "LKLeastSquares", "extract_snapshots": "ExtractSnapshots",
     "cgemm_batch": "CgemmBatch", "solve_batch": "SolveBatch", "gamma_weights": "GammaWeights",
     "inner_products": "InnerProducts",
 }
@@ -388,6 +389,18 @@ class Emitter:
             return self._hs_matrix_powers(n, where)
         if fname == "warp_img2d":
             return self._warp_img2d(n, where)
+        if fname in ("set_brox_matrix", "set_brox_redblack_matrix"):
+            return self._set_brox_matrix(n, where, fname == "set_brox_redblack_matrix")
+        if fname == "applyBroxPreconditioner":
+            return self._apply_brox_preconditioner(n, where)
+        if fname in ("red", "black"):
+            args = self._call_args(n, where, ["Array2D"])
+            dst = self.create_copy(Array2D, _unique_name(), args[0])
+            m = Filter("RedArray2D" if fname == "red" else "BlackArray2D")
+            m.children += [dst] + args
+            self.vars[id(n)] = dst
+            self.nodes[id(n)] = self.nodes.get(id(n.args[0]), []) + [m]
+            return
         rules = {("sum", "Array"): ("ArraySum", Scalar), ("sum2d", "Array2D"): ("Array2DSum", Scalar),
                  ("sqrt", "Scalar"): ("ScalarSqrt", Scalar), ("sqrt", "Array"): ("ArraySqrt", Array),
                  ("sqrt", "Array2D"): ("Array2DSqrt", Array2D)}
@@ -433,6 +446,38 @@ class Emitter:
         y0 = self.create_copy(Array2D, _unique_name(), args[4])
         y1 = self.create_copy(Array2D, _unique_name(), args[5])
         m = Filter("ApplyHSPreconditioner")
+        m.children += [y0, y1] + args
+        self.vars[id(n)] = [y0, y1]
+        self.nodes[id(n)] = list(itertools.chain.from_iterable(
+            self.nodes.get(id(a), []) for a in n.args)) + [m]
+
+    def _set_brox_matrix(self, n, where, redblack):
+        """Templates.py:311-350: set_brox_matrix(A, psi, alpha) as a statement (it
+        writes A's planes), or diag = set_brox_redblack_matrix(A, psi, alpha)."""
+        args = self._call_args(n, where, ["SpMat2D", "Array2D", "Scalar"])
+        A = self.symtab[args[0].properties["name"]].properties
+        psi = self.symtab[args[1].properties["name"]].properties
+        if (str(A["dim0"]), str(A["dim1"])) != (str(psi["dim0"]), str(psi["dim1"])):
+            raise HindemithError(f"{where}: {n.func.id}: matrix planes and psi differ in shape")
+        m = Filter("SetBroxRedblackMatrix" if redblack else "SetBroxMatrix")
+        if redblack:
+            diag = self.create_copy(Array2D, _unique_name(), args[1])
+            m.children.append(diag)
+            self.vars[id(n)] = diag
+        else:
+            self.vars[id(n)] = None
+        m.children += args
+        self.nodes[id(n)] = list(itertools.chain.from_iterable(
+            self.nodes.get(id(a), []) for a in n.args)) + [m]
+
+    def _apply_brox_preconditioner(self, n, where):
+        """Templates.py:375-400: (y0, y1) = applyBroxPreconditioner(A, diag0, diag1,
+        offdiag, x0, x1) with A an SpMat2D whose plane 2 is the diagonal."""
+        args = self._call_args(n, where, ["SpMat2D"] + ["Array2D"] * 5)
+        self._same_shape(args[1:], where)
+        y0 = self.create_copy(Array2D, _unique_name(), args[4])
+        y1 = self.create_copy(Array2D, _unique_name(), args[5])
+        m = Filter("ApplyBroxPreconditioner")
         m.children += [y0, y1] + args
         self.vars[id(n)] = [y0, y1]
         self.nodes[id(n)] = list(itertools.chain.from_iterable(
@@ -540,6 +585,9 @@ class Emitter:
     def visit_Expr(self, n):
         if isinstance(n.value, ast.Constant) and isinstance(n.value.value, str):
             self.nodes[id(n)] = []  # docstring
+        elif (isinstance(n.value, ast.Call) and isinstance(n.value.func, ast.Name)
+              and n.value.func.id == "set_brox_matrix"):
+            self.nodes[id(n)] = self.nodes.get(id(n.value), [])  # Templates.py:311-330
 
     def visit_Pass(self, n):
         self.nodes[id(n)] = []

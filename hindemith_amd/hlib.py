@@ -19,6 +19,11 @@ places (both documented divergences of the reference itself):
 * ``hs_matrix_powers`` has no NumPy twin in the reference; this one follows
   HSMatrixPowers.cpp:72-126 and takes the number of powers from
   ``defines['STEPS']`` as the kernel does.
+* ``set_brox_matrix`` stores each coefficient at the neighbour's index and
+  leaves entries no statement writes alone, as the kernel does
+  (SetBroxMatrix.cpp:71-75); the reference's twin stores them at the pixel and
+  zeroes the planes first (hlib.py:71-82), matching its own pixel-indexed
+  SpMat2D product.
 * ``warp_img2d`` splits the offset as the kernel does (truncate, one step down
   when negative, WarpImg2D.cpp:37-50) and copies the source pixel when a tap
   leaves the image; the reference's twin scales the fractions by 1 - sign
@@ -29,7 +34,8 @@ from __future__ import annotations
 import numpy as np
 
 __all__ = ["Stencil", "SpMat2D", "sum2d", "applyHSPreconditioner", "hs_matrix_powers",
-           "warp_img2d"]
+           "warp_img2d", "set_brox_matrix", "set_brox_redblack_matrix",
+           "applyBroxPreconditioner", "red", "black"]
 
 
 class Stencil:
@@ -164,3 +170,63 @@ def warp_img2d(src, offx, offy):
     val = (src[sy, sx] * (1 - xf) * (1 - yf) + src[sy, sx1] * xf * (1 - yf) +
            src[sy1, sx] * (1 - xf) * yf + src[sy1, sx1] * xf * yf)
     return np.where(ok, val, src).astype(src.dtype)
+
+
+def _brox_planes(A, psi, alpha, planes):
+    """Write the psi-weighted Laplacian's off-diagonals into planes (top, left,
+    right, bottom) of A.data, each at the neighbour's index; return the diagonal."""
+    psi = np.asarray(psi)
+    H, W = psi.shape
+    d = A.data
+    t, l, r, b = planes
+    mid = np.zeros((H, W), np.float32)
+    d[t, :-1, :] = -alpha * psi[:-1, :]          # pixel (y, x) with y > 0 writes at y - 1
+    mid[1:, :] += alpha * psi[:-1, :]
+    d[l, :, :-1] = -alpha * psi[:, :-1]
+    mid[:, 1:] += alpha * psi[:, :-1]
+    d[r, :, 1:] = -alpha * psi[:, :-1]
+    mid[:, :-1] += alpha * psi[:, :-1]
+    d[b, 1:, :] = -alpha * psi[:-1, :]
+    mid[:-1, :] += alpha * psi[:-1, :]
+    return mid
+
+
+def set_brox_matrix(A, psi, alpha):
+    """SetBroxMatrix.cpp:34-79: planes 0..4 of A (offsets (0,-1), (-1,0), (0,0),
+    (1,0), (0,1)) become the psi-weighted 4-neighbour Laplacian."""
+    A.data[2] = _brox_planes(A, psi, alpha, (0, 1, 3, 4))
+
+
+def set_brox_redblack_matrix(A, psi, alpha):
+    """SetBroxRedblackMatrix.cpp:37-81: off-diagonals into planes 0..3, the
+    diagonal returned as an array."""
+    return _brox_planes(A, psi, alpha, (0, 1, 2, 3))
+
+
+def applyBroxPreconditioner(A, diag0, diag1, offdiag, x0, x1):
+    """ApplyBroxPreconditioner.cpp:46-72: 2x2 solve with a = diag0 + A[2],
+    d = diag1 + A[2], b = offdiag; x passed through where |det| <= 1e-4."""
+    x0 = np.asarray(x0)
+    a = diag0 + A.data[2]
+    d = diag1 + A.data[2]
+    b = np.asarray(offdiag)
+    det = a * d - b * b
+    ok = np.abs(det.astype(np.float64)) > 1e-4
+    safe = np.where(ok, det, 1)
+    y0 = np.where(ok, (x0 * d - b * x1) / safe, x0)
+    y1 = np.where(ok, (-b * x0 + a * x1) / safe, x1)
+    return y0.astype(x0.dtype), y1.astype(x0.dtype)
+
+
+def red(x):
+    """RedArray2D.cpp:20-24: x where (column + row) is even, else 0."""
+    x = np.asarray(x)
+    rr, cc = np.indices(x.shape)
+    return np.where((rr + cc) & 1, 0, x).astype(x.dtype)
+
+
+def black(x):
+    """BlackArray2D.cpp:20-24: x where (column + row) is odd, else 0."""
+    x = np.asarray(x)
+    rr, cc = np.indices(x.shape)
+    return np.where((rr + cc) & 1, x, 0).astype(x.dtype)

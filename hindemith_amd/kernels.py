@@ -19,7 +19,8 @@ from ._lib import HindemithError, check, lib
 __all__ = [
     "taps_of", "stencil2d", "dia2d_spmv", "csr_spmv", "CSRBins", "csr_spmv_binned",
     "sum2d", "hs_setup", "hs_jacobi", "hs_solve", "hs_tune", "hs_variants", "stream_ptr",
-    "hs_precondition", "hs_matrix_powers", "warp_img2d", "dia_spmv",
+    "hs_precondition", "hs_matrix_powers", "warp_img2d", "dia_spmv", "brox_set_matrix",
+    "brox_set_redblack_matrix", "brox_precondition",
 ]
 
 
@@ -293,3 +294,47 @@ def dia_spmv(data, offsets, x, out=None):
     check(lib().hmx_dia_spmv(_ptr(data), nd, off.ctypes.data_as(ctypes.c_void_p), _ptr(x), _ptr(y),
                              n, stream_ptr()), "dia_spmv")
     return y
+
+
+def brox_set_matrix(A, psi, alpha):
+    """a19: planes 0..4 of the SpMat2D planes A (nd x H x W, in place) become the
+    psi-weighted Laplacian (the reference's set_brox_matrix)."""
+    _dev(A, "A")
+    _dev(psi, "psi")
+    nd, H, W = A.shape
+    if tuple(psi.shape) != (H, W):
+        raise HindemithError("brox_set_matrix: psi must be shaped like the planes")
+    check(lib().hmx_brox_set_matrix(_ptr(A), nd, _ptr(psi), ctypes.c_float(alpha), H, W,
+                                    stream_ptr()), "brox_set_matrix")
+    return A
+
+
+def brox_set_redblack_matrix(A, psi, alpha, out=None):
+    """a19: off-diagonals into planes 0..3 of A (in place); returns the diagonal."""
+    _dev(A, "A")
+    _dev(psi, "psi")
+    nd, H, W = A.shape
+    if tuple(psi.shape) != (H, W):
+        raise HindemithError("brox_set_redblack_matrix: psi must be shaped like the planes")
+    diag = torch.empty_like(psi) if out is None else _dev(out, "out")
+    check(lib().hmx_brox_set_redblack_matrix(_ptr(diag), _ptr(A), nd, _ptr(psi),
+                                             ctypes.c_float(alpha), H, W, stream_ptr()),
+          "brox_set_redblack_matrix")
+    return diag
+
+
+def brox_precondition(A, diag0, diag1, offdiag, x0, x1, out=None):
+    """a19: the reference's applyBroxPreconditioner(A, diag0, diag1, offdiag, x0, x1)."""
+    for n, t in (("A", A), ("diag0", diag0), ("diag1", diag1), ("offdiag", offdiag),
+                 ("x0", x0), ("x1", x1)):
+        _dev(t, n)
+    nd, H, W = A.shape
+    for t in (diag0, diag1, offdiag, x0, x1):
+        if tuple(t.shape) != (H, W):
+            raise HindemithError("brox_precondition: planes must be shaped like the matrix")
+    y0, y1 = (torch.empty_like(x0), torch.empty_like(x0)) if out is None else out
+    check(lib().hmx_brox_precondition(_ptr(A), nd, _ptr(diag0), _ptr(diag1), _ptr(offdiag),
+                                      _ptr(x0), _ptr(x1), _ptr(_dev(y0, "y0")),
+                                      _ptr(_dev(y1, "y1")), H, W, stream_ptr()),
+          "brox_precondition")
+    return y0, y1

@@ -155,7 +155,9 @@ enum hmx_op {
     HMX_OP_DIV = 8,
     HMX_OP_SQRT = 9,    /* r[dst] = sqrt(r[a]) */
     HMX_OP_STORE = 10,  /* out[dst][i] = r[a] */
-    HMX_OP_REDUCE = 11  /* reduction slot dst += r[a] */
+    HMX_OP_REDUCE = 11, /* reduction slot dst += r[a] */
+    HMX_OP_RED = 12,    /* r[dst] = (x + y) odd ? 0 : r[a]   (RedArray2D.cpp:20-24) */
+    HMX_OP_BLACK = 13   /* r[dst] = (x + y) odd ? r[a] : 0   (BlackArray2D.cpp:20-24) */
 };
 
 /* a10. Canonical HS setup: Ix = Dx*I1 + Dx*I2, Iy = Dy*I1 + Dy*I2,
@@ -327,6 +329,25 @@ int hmx_hs_precondition(const float *diag0, const float *diag1, const float *off
 int hmx_hs_matrix_powers(float lam, const float *Ix, const float *Iy, const float *r0,
                          const float *r1, float *V0, float *V1, int H, int W, int steps,
                          void *stream);
+
+/* a19. Brox SOR building blocks (device pointers, fp32, H x W planes).
+ * hmx_brox_set_matrix: planes 0..4 of A (offsets (0,-1), (-1,0), (0,0), (1,0),
+ * (0,1)) get the psi-weighted 4-neighbour Laplacian, each coefficient at the
+ * neighbour's index; replaces SetBroxMatrix (SetBroxMatrix.cpp:34-79).
+ * hmx_brox_set_redblack_matrix: the off-diagonals into planes 0..3 (top, left,
+ * right, bottom) and the diagonal into Adiag; replaces SetBroxRedblackMatrix
+ * (SetBroxRedblackMatrix.cpp:37-81).
+ * hmx_brox_precondition: per-pixel 2x2 solve with a = diag0 + A[2], d = diag1 +
+ * A[2], b = offdiag, x passed through where |det| <= 1e-4; replaces
+ * ApplyBroxPreconditioner (ApplyBroxPreconditioner.cpp:46-72).  nd: A's plane
+ * count. */
+int hmx_brox_set_matrix(float *A, int nd, const float *psi, float alpha, int H, int W,
+                        void *stream);
+int hmx_brox_set_redblack_matrix(float *Adiag, float *A, int nd, const float *psi, float alpha,
+                                 int H, int W, void *stream);
+int hmx_brox_precondition(const float *A, int nd, const float *diag0, const float *diag1,
+                          const float *offdiag, const float *x0, const float *x1, float *y0,
+                          float *y1, int H, int W, void *stream);
 
 /* 1-D DIA SpMV (scipy dia_matrix, square n x n, data nd x n):
  * y[i] = sum_d [0 <= i+off_d < n] A[d][i+off_d] * x[i+off_d], fma'd in

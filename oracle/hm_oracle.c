@@ -631,3 +631,74 @@ void or_warp_img2d_f64(const double *src, const double *offx, const double *offy
             }
         }
 }
+
+/* ------------------------------------------------------------------------
+ * Brox SOR building blocks.
+ * or_brox_set_f32: SetBroxMatrix.cpp:34-79 (rb = 0: planes top, left, mid,
+ * right, bottom) and SetBroxRedblackMatrix.cpp:37-81 (rb = 1: planes top, left,
+ * right, bottom; the diagonal into diag).  Off-diagonal coefficients at the
+ * neighbour's index: -alpha*psi_e; the diagonal accumulates alpha*psi_e (fma)
+ * in the order up, left, right, down.  Entries no statement writes keep their
+ * value.
+ */
+void or_brox_set_f32(float *A, float *diag, const float *psi, int H, int W, float alpha, int rb)
+{
+    const int64_t HW = (int64_t)H * W;
+    const int pt = 0, pl = 1, pr = rb ? 2 : 3, pb = rb ? 3 : 4;
+    const float nalpha = -alpha;
+    for (int r = 0; r < H; ++r)
+        for (int c = 0; c < W; ++c) {
+            const int64_t p = (int64_t)r * W + c;
+            float mid = 0.0f;
+            if (r > 0) { A[pt * HW + p - W] = nalpha * psi[p - W]; mid = fmaf(alpha, psi[p - W], mid); }
+            if (c > 0) { A[pl * HW + p - 1] = nalpha * psi[p - 1]; mid = fmaf(alpha, psi[p - 1], mid); }
+            if (c < W - 1) { A[pr * HW + p + 1] = nalpha * psi[p]; mid = fmaf(alpha, psi[p], mid); }
+            if (r < H - 1) { A[pb * HW + p + W] = nalpha * psi[p]; mid = fmaf(alpha, psi[p], mid); }
+            if (rb) diag[p] = mid;
+            else A[2 * HW + p] = mid;
+        }
+}
+
+/* or_brox_precond_*: ApplyBroxPreconditioner.cpp:46-72: a = diag0 + A2, d = diag1
+ * + A2 (A2 = the matrix's plane 2 at the pixel), threshold |det| > 1e-4. */
+void or_brox_precond_f32(const float *A2, const float *diag0, const float *diag1,
+                         const float *offdiag, const float *x0, const float *x1, float *y0,
+                         float *y1, int64_t n)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < n; ++p) {
+        const float a = diag0[p] + A2[p], d = diag1[p] + A2[p];
+        const float b = offdiag[p], e = x0[p], f = x1[p];
+        const float ad = a * d, bc = b * b;
+        const float det = ad - bc;
+        if (fabs((double)det) > 1e-4) {
+            const float t0 = e * d, t1 = b * f;
+            const float n0 = t0 - t1;
+            const float t2 = (-b) * e, t3 = a * f;
+            const float n1 = t2 + t3;
+            y0[p] = n0 / det;
+            y1[p] = n1 / det;
+        } else {
+            y0[p] = e;
+            y1[p] = f;
+        }
+    }
+}
+
+void or_brox_precond_f64(const double *A2, const double *diag0, const double *diag1,
+                         const double *offdiag, const double *x0, const double *x1, double *y0,
+                         double *y1, int64_t n)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < n; ++p) {
+        const double a = diag0[p] + A2[p], d = diag1[p] + A2[p], b = offdiag[p];
+        const double det = a * d - b * b;
+        if (fabs(det) > 1e-4) {
+            y0[p] = (x0[p] * d - b * x1[p]) / det;
+            y1[p] = (-b * x0[p] + a * x1[p]) / det;
+        } else {
+            y0[p] = x0[p];
+            y1[p] = x1[p];
+        }
+    }
+}

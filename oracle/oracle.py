@@ -16,6 +16,7 @@ __all__ = [
     "sum2d", "hs_setup", "hs_jacobi", "hs_solve", "hs_grad", "num_threads",
     "set_num_threads", "np_stencil2d", "np_dia2d_csr", "np_hs_solve_f64",
     "endpoint_error", "hs_precondition", "hs_matrix_powers", "warp_img2d", "dia1d",
+    "brox_set_matrix", "brox_precondition", "red", "black",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -315,6 +316,45 @@ def warp_img2d(src, offx, offy, dtype=np.float32):
     fn = lib.or_warp_img2d_f32 if dtype == np.float32 else lib.or_warp_img2d_f64
     fn(_p(s), _p(ox), _p(oy), _p(d), H, W)
     return d
+
+
+def brox_set_matrix(A, psi, alpha, redblack=False):
+    """SetBroxMatrix.cpp:34-79 / SetBroxRedblackMatrix.cpp:37-81 (fp32): returns the
+    updated planes (a copy of A) and, for the red-black form, the diagonal."""
+    lib = load_oracle()
+    A = _f32(A).copy()
+    psi = _f32(psi)
+    H, W = psi.shape
+    diag = np.zeros((H, W), np.float32)
+    lib.or_brox_set_f32(_p(A), _p(diag), _p(psi), H, W, ctypes.c_float(alpha),
+                        1 if redblack else 0)
+    return (A, diag) if redblack else A
+
+
+def brox_precondition(A, diag0, diag1, offdiag, x0, x1, dtype=np.float32):
+    """ApplyBroxPreconditioner.cpp:46-72 with A2 = plane 2 of the matrix A."""
+    lib = load_oracle()
+    cast = _f32 if dtype == np.float32 else _f64
+    a2 = cast(np.asarray(A)[2])
+    d0, d1, od, e, f = (cast(t) for t in (diag0, diag1, offdiag, x0, x1))
+    y0, y1 = np.empty_like(e), np.empty_like(e)
+    fn = lib.or_brox_precond_f32 if dtype == np.float32 else lib.or_brox_precond_f64
+    fn(_p(a2), _p(d0), _p(d1), _p(od), _p(e), _p(f), _p(y0), _p(y1), ctypes.c_int64(e.size))
+    return y0, y1
+
+
+def red(x):
+    """RedArray2D.cpp:20-24: x where (column + row) is even, else 0."""
+    x = np.asarray(x)
+    rr, cc = np.indices(x.shape)
+    return np.where((rr + cc) & 1, 0, x).astype(x.dtype)
+
+
+def black(x):
+    """BlackArray2D.cpp:20-24: x where (column + row) is odd, else 0."""
+    x = np.asarray(x)
+    rr, cc = np.indices(x.shape)
+    return np.where((rr + cc) & 1, x, 0).astype(x.dtype)
 
 
 def endpoint_error(u, v, fx, fy, margin=8):

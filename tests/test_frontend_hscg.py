@@ -144,3 +144,53 @@ def test_dia_matrix_must_be_square():
     A = sp.dia_matrix((np.ones((1, 6), np.float32), np.array([0])), shape=(4, 6))
     with pytest.raises(HindemithError, match="square"):
         f.compile(A=A, x=np.zeros(6, np.float32))
+
+
+BROX_OFFX = np.array([0, -1, 0, 1, 0], np.int32)
+BROX_OFFY = np.array([-1, 0, 0, 0, 1], np.int32)
+
+
+@hm
+def brox_sweep(A, psi, alpha, du, b0, diag0, diag1, offdiag):
+    set_brox_matrix(A, psi, alpha)  # noqa: F821 (resolved by the @hm front end)
+    for i in range(2):
+        r0 = b0 - A * du
+        z0, z1 = applyBroxPreconditioner(A, diag0, diag1, offdiag, r0, r0)  # noqa: F821
+        du = du + red(z0)  # noqa: F821
+        r0 = b0 - A * du
+        z0, z1 = applyBroxPreconditioner(A, diag0, diag1, offdiag, r0, r0)  # noqa: F821
+        du = du + black(z0)  # noqa: F821
+    return du
+
+
+def brox_kw(H=10, W=14):
+    from hindemith_amd.hindemith import SpMat2D
+    rng = np.random.default_rng(0)
+    A = SpMat2D(np.zeros((5, H, W), np.float32), BROX_OFFX, BROX_OFFY)
+    a = [rng.uniform(0.5, 1.5, (H, W)).astype(np.float32) for _ in range(6)]
+    return dict(A=A, psi=a[0], alpha=np.float32(0.5), du=a[1], b0=a[2], diag0=a[3],
+                diag1=a[4], offdiag=a[5] * 0.1)
+
+
+def test_brox_program_plans():
+    plan = brox_sweep.compile(**brox_kw())
+    assert plan.startswith("brox_set_matrix A from psi=psi")
+    assert plan.count("brox_precondition (") == 2 and "loop x2" in plan
+
+
+def test_brox_twins_follow_the_oracle():
+    from hindemith_amd.hindemith import (applyBroxPreconditioner, black, red,
+                                         set_brox_matrix, set_brox_redblack_matrix)
+    k = brox_kw()
+    A = k["A"]
+    set_brox_matrix(A, k["psi"], 0.5)
+    ref = oracle.brox_set_matrix(np.zeros_like(A.data), k["psi"], 0.5)
+    assert np.allclose(A.data, ref, rtol=1e-6, atol=1e-6)
+    diag = set_brox_redblack_matrix(A, k["psi"], 0.5)
+    assert np.allclose(diag, ref[2], rtol=1e-6, atol=1e-6)
+    y0, y1 = applyBroxPreconditioner(A, k["diag0"], k["diag1"], k["offdiag"], k["du"], k["b0"])
+    r0, r1 = oracle.brox_precondition(A.data, k["diag0"], k["diag1"], k["offdiag"], k["du"],
+                                      k["b0"], np.float64)
+    assert np.allclose(y0, r0, rtol=1e-5, atol=1e-5) and np.allclose(y1, r1, rtol=1e-5, atol=1e-5)
+    assert np.array_equal(red(k["du"]), oracle.red(k["du"]))
+    assert np.array_equal(black(k["du"]), oracle.black(k["du"]))

@@ -78,7 +78,11 @@ __device__ __forceinline__ float hs_update(float ix, float iy, float it, float d
 // DB: uv holds two windows; step j reads window (j-1)&1 and writes its result
 // straight into the other, so a step needs one barrier and no new values are
 // held in registers across it.
-template <int NT, int TW, int TH, int KMAX, int NTAP, bool LOUT = false, bool DB = false>
+// SHARED: the tap addresses are per-thread values shared by all owned cells
+// (one register per tap, the cell a constant offset); otherwise each cell
+// computes its own, for builds with no registers to spare.
+template <int NT, int TW, int TH, int KMAX, int NTAP, bool LOUT = false, bool DB = false,
+          bool SHARED = !LOUT>
 __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
                                         const HsTapTable<TW + 2 * KMAX> &tp, float2 *uv,
                                         float *lin = nullptr) {
@@ -92,21 +96,32 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
     const int x1 = min(x0 + TW, a.W), y1 = min(y0 + TH, a.H);
     const int gx0 = max(x0 - k, 0), gy0 = max(y0 - k, 0);
     const int gx1 = min(x1 + k, a.W), gy1 = min(y1 + k, a.H);
-    const int RW = gx1 - gx0, RH = gy1 - gy0, npx = RW * RH;
+    const int RW = gx1 - gx0, RH = gy1 - gy0;
     const bool clipped = (x0 - k < 0) || (y0 - k < 0) || (x1 + k > a.W) || (y1 + k > a.H);
 
+    // Thread tid owns the window cells c = tid + j*NT (j < P), so the LDS
+    // address of tap d of any owned pixel is (tid + dl[d]) + j*NT: one
+    // per-thread base per tap, the slot a compile-time offset.  pk packs, for
+    // a cell inside the region, L << 27 | ry << 16 | rx, where L = k - (the
+    // cell's distance outside the output tile) is the last step that updates
+    // it: the region of step it is the tile grown by k - it, clipped to the
+    // image, so the cell is updated exactly when it <= L.  Cells outside the
+    // region hold -1.
     float cx[P], cy[P], ct[P], cd[P];
-    int pk[P];  // (ry << 16) | rx of each owned region pixel
+    int pk[P];
 #pragma unroll
     for (int j = 0; j < P; ++j) {
-        const int i = tid + j * NT;
+        const int c = tid + j * NT;
+        const int ry = c / PITCH, rx = c - ry * PITCH;
         pk[j] = -1;
         cx[j] = cy[j] = ct[j] = 0.0f;
         cd[j] = 1.0f;
-        if (i < npx) {
-            const int ry = i / RW, rx = i - ry * RW;
-            pk[j] = (ry << 16) | rx;
-            const int64_t g = (int64_t)(gy0 + ry) * a.W + gx0 + rx;
+        if (c < NPX && rx < RW && ry < RH) {
+            const int gx = gx0 + rx, gy = gy0 + ry;
+            const int dx = max(max(x0 - gx, gx - x1 + 1), 0);
+            const int dy = max(max(y0 - gy, gy - y1 + 1), 0);
+            pk[j] = ((k - max(dx, dy)) << 27) | (ry << 16) | rx;
+            const int64_t g = (int64_t)gy * a.W + gx;
             const float ix = a.Ix[g], iy = a.Iy[g];
             cx[j] = ix;
             cy[j] = iy;
@@ -119,32 +134,29 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
                 const float m2 = iy * iy;
                 cd[j] = s1 + m2;
             }
-            uv[ry * PITCH + rx] = make_float2(a.u_in[g], a.v_in[g]);
+            uv[c] = make_float2(a.u_in[g], a.v_in[g]);
         }
     }
     __syncthreads();
 
     for (int it = 1; it <= k; ++it) {
-        const int e = k - it;
-        const int rx0 = max(x0 - e, 0) - gx0, ry0 = max(y0 - e, 0) - gy0;
-        const int rx1 = min(x1 + e, a.W) - gx0, ry1 = min(y1 + e, a.H) - gy0;
+        const int act_min = it << 27;
         const float2 *uvr = DB ? uv + ((it - 1) & 1) * NPX : uv;
         float2 *uvw = DB ? uv + (it & 1) * NPX : uv;
         float nu[P], nv[P];
         bool act[P];
 #pragma unroll
         for (int j = 0; j < P; ++j) {
-            // Opaque copy: stops the compiler from hoisting every pixel's tap
-            // addresses out of the iteration loop (P x taps live registers).
-            int pkj = pk[j];
-            asm volatile("" : "+v"(pkj));
-            const int rx = pkj & 0xffff, ry = pkj >> 16;
-            act[j] = pkj >= 0 && rx >= rx0 && rx < rx1 && ry >= ry0 && ry < ry1;
-            nu[j] = nv[j] = 0.0f;
+            const int pkj = pk[j];
+            act[j] = pkj >= act_min;
             if (act[j]) {
                 float ub = 0.0f, vb = 0.0f;
                 if (!clipped) {
-                    const float2 *base = uvr + ry * PITCH + rx;
+                    // an updated cell is at least one cell inside the window
+                    // on every side, so every tap stays inside it
+                    int c = tid + j * NT;
+                    if (!SHARED) asm volatile("" : "+v"(c));
+                    const float2 *base = uvr + c;
 #pragma unroll
                     for (int d = 0; d < (NTAP ? NTAP : HS_MAX_TAPS); ++d) {
                         if (NTAP || d < tp.nd) {
@@ -154,6 +166,12 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
                         }
                     }
                 } else {
+                    // Opaque copy: stops the compiler from hoisting every
+                    // pixel's clamped tap addresses out of the step loop (P x
+                    // taps live registers).
+                    int pkc = pkj;
+                    asm volatile("" : "+v"(pkc));
+                    const int rx = pkc & 0xffff, ry = (pkc >> 16) & 0x7ff;
                     const int gx = gx0 + rx, gy = gy0 + ry;
 #pragma unroll
                     for (int d = 0; d < (NTAP ? NTAP : HS_MAX_TAPS); ++d) {
@@ -167,8 +185,9 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
                     }
                 }
                 const float tt = hs_update(cx[j], cy[j], ct[j], cd[j], ub, vb, nu[j], nv[j]);
-                if (DB) uvw[ry * PITCH + rx] = make_float2(nu[j], nv[j]);
+                if (DB) uvw[tid + j * NT] = make_float2(nu[j], nv[j]);
                 if (LOUT) {
+                    const int rx = pkj & 0xffff, ry = (pkj >> 16) & 0x7ff;
                     const int lx = gx0 + rx - x0, ly = gy0 + ry - y0;
                     if (lx >= 0 && lx < TW && ly >= 0 && ly < TH) {
                         const int li = ly * TW + lx;
@@ -188,7 +207,7 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < P; ++j)
-            if (act[j]) uv[(pk[j] >> 16) * PITCH + (pk[j] & 0xffff)] = make_float2(nu[j], nv[j]);
+            if (act[j]) uv[tid + j * NT] = make_float2(nu[j], nv[j]);
         if (LOUT) {
             // lin is rewritten only in the next step's compute, after the
             // barrier below
@@ -209,11 +228,11 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
 #pragma unroll
     for (int j = 0; j < P; ++j) {
         if (pk[j] < 0) continue;
-        const int rx = pk[j] & 0xffff, ry = pk[j] >> 16;
+        const int rx = pk[j] & 0xffff, ry = (pk[j] >> 16) & 0x7ff;
         const int gx = gx0 + rx, gy = gy0 + ry;
         if (gx >= x0 && gx < x1 && gy >= y0 && gy < y1) {
             const int64_t g = (int64_t)gy * a.W + gx;
-            const float2 q = (DB ? uv + (k & 1) * NPX : uv)[ry * PITCH + rx];
+            const float2 q = (DB ? uv + (k & 1) * NPX : uv)[tid + j * NT];
             a.u_out[g] = q.x;
             a.v_out[g] = q.y;
         }

@@ -66,9 +66,22 @@ __device__ __forceinline__ float hs_update(float ix, float iy, float it, float d
     return tt;
 }
 
+// LDS of one tile window (two with DB): PAD cells before and after it, so a
+// thread may read the taps of any cell it owns, updated or not, without
+// leaving the allocation.  hs_tile takes uv = the allocation + PAD.
+template <int NT, int TW, int TH, int KMAX, bool DB>
+struct HsWindow {
+    static constexpr int PITCH = TW + 2 * KMAX;
+    static constexpr int NPX = PITCH * (TH + 2 * KMAX);
+    static constexpr int OWNED = (NPX + NT - 1) / NT * NT;  // cells the threads own
+    static constexpr int PAD = PITCH + 1;
+    static constexpr int SIZE = PAD + (DB ? NPX : 0) + OWNED + PAD;
+};
+
 // Advance output tile t of image a by k (<= KMAX) Jacobi steps.  uv is the
-// work-group's LDS window of (TW + 2*KMAX) x (TH + 2*KMAX) float2, stored with
-// the fixed row pitch TW + 2*KMAX so every tap is a constant LDS delta.
+// work-group's LDS window of (TW + 2*KMAX) x (TH + 2*KMAX) float2 (padded as
+// HsWindow), stored with the fixed row pitch TW + 2*KMAX so every tap is a
+// constant LDS delta.
 // NTAP > 0: the stencil has exactly NTAP taps (straight-line code, batched LDS
 // reads); NTAP == 0: any count up to HS_MAX_TAPS, tested per tap.
 // LOUT: also write each step's (t, ubar, vbar) of the output tile to a.lt,
@@ -80,13 +93,20 @@ __device__ __forceinline__ float hs_update(float ix, float iy, float it, float d
 // held in registers across it.
 // SHARED: the tap addresses are per-thread values shared by all owned cells
 // (one register per tap, the cell a constant offset); otherwise each cell
-// computes its own, for builds with no registers to spare.
+// computes its own, for builds with no registers to spare (LOUT, and more
+// than 8 cells per thread).
+// PF: read the next cell's taps ahead (see the step loop); needs SHARED, a
+// fixed tap count and no LOUT.  On by default where it fits the registers of
+// two 1024-thread work-groups per CU: the double-buffered builds, whose
+// second barrier's held results free the registers it needs.
 template <int NT, int TW, int TH, int KMAX, int NTAP, bool LOUT = false, bool DB = false,
-          bool SHARED = !LOUT>
+          bool SHARED = !LOUT && ((TW + 2 * KMAX) * (TH + 2 * KMAX) + NT - 1) / NT <= 8,
+          bool PF = SHARED && DB>
 __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
                                         const HsTapTable<TW + 2 * KMAX> &tp, float2 *uv,
                                         float *lin = nullptr) {
     static_assert(!(LOUT && DB), "the linearisation output needs the two-barrier step");
+    constexpr bool PREFETCH = PF && SHARED && NTAP > 0 && !LOUT;
     constexpr int PITCH = TW + 2 * KMAX, RHM = TH + 2 * KMAX;
     constexpr int NPX = PITCH * RHM;
     constexpr int P = (NPX + NT - 1) / NT;
@@ -145,6 +165,41 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
         float2 *uvw = DB ? uv + (it & 1) * NPX : uv;
         float nu[P], nv[P];
         bool act[P];
+        if (PREFETCH && !clipped) {
+            // Software-pipelined form: the first NPF taps of cell j+1 are read
+            // (updated or not: the window is padded) before cell j is
+            // computed, so LDS latency overlaps arithmetic instead of stalling
+            // the wave; the rest of a cell's taps are read as it starts.
+            constexpr int NPF = PREFETCH ? (NTAP + 1) / 2 : 1;
+            const float2 *b0 = uvr + tid;
+            float2 qa[NPF];
+#pragma unroll
+            for (int d = 0; d < NPF; ++d) qa[d] = b0[tp.dl[d]];
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                float2 qr[PREFETCH ? NTAP - NPF + 1 : 1], qn[NPF];
+#pragma unroll
+                for (int d = NPF; d < NTAP; ++d) qr[d - NPF] = b0[j * NT + tp.dl[d]];
+                if (j + 1 < P) {
+#pragma unroll
+                    for (int d = 0; d < NPF; ++d) qn[d] = b0[(j + 1) * NT + tp.dl[d]];
+                }
+                act[j] = pk[j] >= act_min;
+                if (act[j]) {
+                    float ub = 0.0f, vb = 0.0f;
+#pragma unroll
+                    for (int d = 0; d < NTAP; ++d) {
+                        const float2 qd = d < NPF ? qa[d] : qr[d - NPF];
+                        ub = fmaf(tp.w[d], qd.x, ub);
+                        vb = fmaf(tp.w[d], qd.y, vb);
+                    }
+                    (void)hs_update(cx[j], cy[j], ct[j], cd[j], ub, vb, nu[j], nv[j]);
+                    if (DB) uvw[tid + j * NT] = make_float2(nu[j], nv[j]);
+                }
+#pragma unroll
+                for (int d = 0; d < NPF; ++d) qa[d] = qn[d];
+            }
+        } else {
 #pragma unroll
         for (int j = 0; j < P; ++j) {
             const int pkj = pk[j];
@@ -197,6 +252,7 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
                     }
                 }
             }
+        }
         }
         if (DB) {
             // the window written here was last read in step it-1, before the

@@ -106,7 +106,9 @@ __device__ inline float setup_tap(const float *__restrict__ x, int H, int W, int
 template <int NT, int TW, int TH, int KMAX, int NTAP, int WPE, bool DB = false>
 __global__ __launch_bounds__(NT, WPE) void hs_batched_kernel(BatchArgs b, SetupTaps st,
                                                              Taps taps) {
-    __shared__ float2 uv[(DB ? 2 : 1) * (TW + 2 * KMAX) * (TH + 2 * KMAX)];
+    using Win = HsWindow<NT, TW, TH, KMAX, DB>;
+    __shared__ float2 uvm[Win::SIZE];
+    float2 *const uv = uvm + Win::PAD;
     __shared__ int s_claim;
     const HsTapTable<TW + 2 * KMAX> tp(taps);
     unsigned arrivals = 0;
@@ -170,7 +172,7 @@ __global__ __launch_bounds__(NT, WPE) void hs_batched_kernel(BatchArgs b, SetupT
             im.W = P.W;
             im.tiles_x = P.tiles_x;
             // the double-buffered builds have registers for shared tap addresses
-            hs_tile<NT, TW, TH, KMAX, NTAP, false, DB, DB>(im, w - P.tile_base, kk, tp, uv);
+            hs_tile<NT, TW, TH, KMAX, NTAP, false, DB, DB, false>(im, w - P.tile_base, kk, tp, uv);
         }
         arrivals += gridDim.x;
         if (!grid_sync(b.barrier, arrivals, b.status, b.spin_limit)) return;

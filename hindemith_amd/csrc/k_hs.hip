@@ -48,7 +48,8 @@ struct HsArgs {
 // DB: two LDS windows, one barrier per step (see hs_tile).
 template <int NT, int TW, int TH, int KMAX, int NTAP, bool LOUT = false, bool DB = false>
 __global__ __launch_bounds__(NT, LOUT ? 8 : 1) void hs_tb_kernel(HsArgs a, Taps taps) {
-    __shared__ float2 uv[(DB ? 2 : 1) * (TW + 2 * KMAX) * (TH + 2 * KMAX)];
+    using Win = HsWindow<NT, TW, TH, KMAX, DB>;
+    __shared__ float2 uvm[Win::SIZE];
     __shared__ float lin[LOUT ? 3 * TW * TH : 1];
     const int64_t off = (int64_t)blockIdx.y * a.bstride;  // pair of a batch
     HsImage im;
@@ -71,8 +72,8 @@ __global__ __launch_bounds__(NT, LOUT ? 8 : 1) void hs_tb_kernel(HsArgs a, Taps 
         im.lstride = a.lstride;
     }
     const HsTapTable<TW + 2 * KMAX> tp(taps);
-    hs_tile<NT, TW, TH, KMAX, NTAP, LOUT, DB>(im, xcd_remap(blockIdx.x, a.ntiles), a.k, tp, uv,
-                                              LOUT ? lin : nullptr);
+    hs_tile<NT, TW, TH, KMAX, NTAP, LOUT, DB>(im, xcd_remap(blockIdx.x, a.ntiles), a.k, tp,
+                                              uvm + Win::PAD, LOUT ? lin : nullptr);
 }
 
 template <int NT, int TW, int TH, int KMAX, int S, int NTAP>

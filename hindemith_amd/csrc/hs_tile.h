@@ -3,7 +3,10 @@
 // launch (k_batched.hip) so both run the same per-pixel code.
 #pragma once
 
+#include <type_traits>
+
 #include "hmx_common.h"
+#include "hs_div.h"
 
 #pragma clang fp contract(off)
 
@@ -52,13 +55,25 @@ struct HsTapTable {
 
 // The per-pixel update of the canonical program, one rounding per op:
 //   t = ((Ix*ubar + Iy*vbar) + It) / den;  u' = ubar - Ix*t;  v' = vbar - Iy*t
-__device__ __forceinline__ float hs_update(float ix, float iy, float it, float den, float ub,
-                                           float vb, float &un, float &vn) {
+// FDIV: the quotient comes from rcp = hs_recip(den) by hs_div_fast (the same
+// correctly rounded value), with the IEEE division for the lanes it declines.
+template <bool FDIV = false>
+__device__ __forceinline__ float hs_update(float ix, float iy, float it, float den, float rcp,
+                                           float ub, float vb, float &un, float &vn) {
     const float m1 = ix * ub;
     const float m2 = iy * vb;
     const float s1 = m1 + m2;
     const float s2 = s1 + it;
-    const float tt = s2 / den;
+    float tt;
+    if (!FDIV) {
+        tt = s2 / den;
+    } else if (!hs_div_fast(s2, den, rcp, tt)) {
+        // opaque operand: keeps the division inside the (rarely taken)
+        // branch instead of computed for every lane and selected
+        float s2o = s2;
+        asm volatile("" : "+v"(s2o));
+        tt = s2o / den;
+    }
     const float p1 = ix * tt;
     const float p2 = iy * tt;
     un = ub - p1;
@@ -95,13 +110,16 @@ struct HsWindow {
 // (one register per tap, the cell a constant offset); otherwise each cell
 // computes its own, for builds with no registers to spare (LOUT, and more
 // than 8 cells per thread).
+// FDIV: divide by a per-cell reciprocal (hs_update), one more register per
+// cell.
 // PF: read the next cell's taps ahead (see the step loop); needs SHARED, a
 // fixed tap count and no LOUT.  On by default where it fits the registers of
-// two 1024-thread work-groups per CU: the double-buffered builds, whose
-// second barrier's held results free the registers it needs.
+// two 1024-thread work-groups per CU: the double-buffered builds without
+// FDIV, whose second barrier's held results free the registers it needs.
 template <int NT, int TW, int TH, int KMAX, int NTAP, bool LOUT = false, bool DB = false,
+          bool FDIV = false,
           bool SHARED = !LOUT && ((TW + 2 * KMAX) * (TH + 2 * KMAX) + NT - 1) / NT <= 8,
-          bool PF = SHARED && DB>
+          bool PF = SHARED && DB && !FDIV>
 __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
                                         const HsTapTable<TW + 2 * KMAX> &tp, float2 *uv,
                                         float *lin = nullptr) {
@@ -127,11 +145,12 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
     // it: the region of step it is the tile grown by k - it, clipped to the
     // image, so the cell is updated exactly when it <= L.  Cells outside the
     // region hold -1.
-    float cx[P], cy[P], ct[P], cd[P];
+    float cx[P], cy[P], ct[P], cd[P], cr[FDIV ? P : 1];
     int pk[P];
 #pragma unroll
     for (int j = 0; j < P; ++j) {
         const int c = tid + j * NT;
+        if (FDIV) cr[j] = 1.0f;
         const int ry = c / PITCH, rx = c - ry * PITCH;
         pk[j] = -1;
         cx[j] = cy[j] = ct[j] = 0.0f;
@@ -154,15 +173,19 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
                 const float m2 = iy * iy;
                 cd[j] = s1 + m2;
             }
+            if (FDIV) cr[j] = hs_recip(cd[j]);
             uv[c] = make_float2(a.u_in[g], a.v_in[g]);
         }
     }
     __syncthreads();
 
-    for (int it = 1; it <= k; ++it) {
+    // One Jacobi step.  PAR (compile time) is the window step it reads with
+    // DB, (it - 1) & 1, so both windows' tap addresses are constants.
+    auto step = [&](const int it, auto par_c) {
+        constexpr int PAR = decltype(par_c)::value;
         const int act_min = it << 27;
-        const float2 *uvr = DB ? uv + ((it - 1) & 1) * NPX : uv;
-        float2 *uvw = DB ? uv + (it & 1) * NPX : uv;
+        const float2 *uvr = DB ? uv + PAR * NPX : uv;
+        float2 *uvw = DB ? uv + (PAR ^ 1) * NPX : uv;
         float nu[P], nv[P];
         bool act[P];
         if (PREFETCH && !clipped) {
@@ -193,7 +216,8 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
                         ub = fmaf(tp.w[d], qd.x, ub);
                         vb = fmaf(tp.w[d], qd.y, vb);
                     }
-                    (void)hs_update(cx[j], cy[j], ct[j], cd[j], ub, vb, nu[j], nv[j]);
+                    (void)hs_update<FDIV>(cx[j], cy[j], ct[j], cd[j], FDIV ? cr[j] : 0.0f, ub,
+                                          vb, nu[j], nv[j]);
                     if (DB) uvw[tid + j * NT] = make_float2(nu[j], nv[j]);
                 }
 #pragma unroll
@@ -239,7 +263,8 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
                         }
                     }
                 }
-                const float tt = hs_update(cx[j], cy[j], ct[j], cd[j], ub, vb, nu[j], nv[j]);
+                const float tt = hs_update<FDIV>(cx[j], cy[j], ct[j], cd[j],
+                                                 FDIV ? cr[j] : 0.0f, ub, vb, nu[j], nv[j]);
                 if (DB) uvw[tid + j * NT] = make_float2(nu[j], nv[j]);
                 if (LOUT) {
                     const int rx = pkj & 0xffff, ry = (pkj >> 16) & 0x7ff;
@@ -258,7 +283,7 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
             // the window written here was last read in step it-1, before the
             // previous barrier
             __syncthreads();
-            continue;
+            return;
         }
         __syncthreads();
 #pragma unroll
@@ -279,6 +304,14 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
             }
         }
         __syncthreads();
+    };
+    if (DB) {
+        for (int it = 1; it <= k; it += 2) {
+            step(it, std::integral_constant<int, 0>());
+            if (it < k) step(it + 1, std::integral_constant<int, 1>());
+        }
+    } else {
+        for (int it = 1; it <= k; ++it) step(it, std::integral_constant<int, 0>());
     }
 
 #pragma unroll
@@ -404,8 +437,8 @@ __device__ __forceinline__ void hs_tile_strip(const HsImage &a, int t, int k,
             }
 #pragma unroll
             for (int s = 0; s < S; ++s)
-                (void)hs_update(cx[m][s], cy[m][s], ct[m][s], cd[m][s], ub[s], vb[s], nu[m][s],
-                          nv[m][s]);
+                (void)hs_update(cx[m][s], cy[m][s], ct[m][s], cd[m][s], 0.0f, ub[s], vb[s],
+                                nu[m][s], nv[m][s]);
         }
         __syncthreads();
 #pragma unroll

@@ -172,7 +172,7 @@ __global__ __launch_bounds__(NT, WPE) void hs_batched_kernel(BatchArgs b, SetupT
             im.W = P.W;
             im.tiles_x = P.tiles_x;
             // the double-buffered builds have registers for shared tap addresses
-            hs_tile<NT, TW, TH, KMAX, NTAP, false, DB, DB, false>(im, w - P.tile_base, kk, tp, uv);
+            hs_tile<NT, TW, TH, KMAX, NTAP, false, DB, false, DB, false>(im, w - P.tile_base, kk, tp, uv);
         }
         arrivals += gridDim.x;
         if (!grid_sync(b.barrier, arrivals, b.status, b.spin_limit)) return;

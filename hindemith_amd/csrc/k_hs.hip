@@ -46,7 +46,8 @@ struct HsArgs {
 // The linearisation-output form (the adjoint's segment recompute) is compiled
 // for 8 waves per SIMD so its extra stores do not cost a work-group per CU.
 // DB: two LDS windows, one barrier per step (see hs_tile).
-template <int NT, int TW, int TH, int KMAX, int NTAP, bool LOUT = false, bool DB = false>
+template <int NT, int TW, int TH, int KMAX, int NTAP, bool LOUT = false, bool DB = false,
+          bool FDIV = false>
 __global__ __launch_bounds__(NT, LOUT ? 8 : 1) void hs_tb_kernel(HsArgs a, Taps taps) {
     using Win = HsWindow<NT, TW, TH, KMAX, DB>;
     __shared__ float2 uvm[Win::SIZE];
@@ -72,8 +73,8 @@ __global__ __launch_bounds__(NT, LOUT ? 8 : 1) void hs_tb_kernel(HsArgs a, Taps 
         im.lstride = a.lstride;
     }
     const HsTapTable<TW + 2 * KMAX> tp(taps);
-    hs_tile<NT, TW, TH, KMAX, NTAP, LOUT, DB>(im, xcd_remap(blockIdx.x, a.ntiles), a.k, tp,
-                                              uvm + Win::PAD, LOUT ? lin : nullptr);
+    hs_tile<NT, TW, TH, KMAX, NTAP, LOUT, DB, FDIV>(im, xcd_remap(blockIdx.x, a.ntiles), a.k,
+                                                    tp, uvm + Win::PAD, LOUT ? lin : nullptr);
 }
 
 template <int NT, int TW, int TH, int KMAX, int S, int NTAP>
@@ -128,6 +129,14 @@ static int tap_class(int nd) {
      {hs_tb_kernel<NT, TW, TH, K, 4, false, true>, hs_tb_kernel<NT, TW, TH, K, 5, false, true>, \
       hs_tb_kernel<NT, TW, TH, K, 8, false, true>, hs_tb_kernel<NT, TW, TH, K, 9, false, true>, \
       hs_tb_kernel<NT, TW, TH, K, 0, false, true>}}
+// quotient from a per-pixel reciprocal (hs_div.h): single or double-buffered window
+#define HS_F(NT, TW, TH, K, DB)                                                         \
+    {NT, TW, TH, K, 0,                                                                  \
+     {hs_tb_kernel<NT, TW, TH, K, 4, false, DB, true>,                                  \
+      hs_tb_kernel<NT, TW, TH, K, 5, false, DB, true>,                                  \
+      hs_tb_kernel<NT, TW, TH, K, 8, false, DB, true>,                                  \
+      hs_tb_kernel<NT, TW, TH, K, 9, false, DB, true>,                                  \
+      hs_tb_kernel<NT, TW, TH, K, 0, false, DB, true>}}
 // column-strip variants (S pixels per thread per column)
 #define HS_S(NT, TW, TH, K, S)                                                          \
     {NT, TW, TH, K, S,                                                                  \
@@ -157,10 +166,16 @@ static const HsVariant kVariants[] = {
     HS_D(1024, 64, 64, 4),    // 19
     HS_D(512, 64, 32, 4),     // 20
     HS_D(1024, 128, 32, 8),   // 21
+    HS_F(1024, 64, 32, 8, false),  // 22: reciprocal quotient
+    HS_F(1024, 64, 32, 4, false),  // 23
+    HS_F(1024, 64, 32, 8, true),   // 24: reciprocal quotient, double-buffered
+    HS_F(1024, 64, 32, 4, true),   // 25
+    HS_F(512, 64, 32, 4, true),    // 26
 };
 #undef HS_V
 #undef HS_D
 #undef HS_S
+#undef HS_F
 static constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
 static std::mutex g_tune_mu;

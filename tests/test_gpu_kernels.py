@@ -233,6 +233,49 @@ def test_hs_jacobi_tap_classes_bitexact(K, kind):
         assert np.array_equal(host(u), ref_u) and np.array_equal(host(v), ref_v), (kind, vi, k)
 
 
+def _bitwise_equal_nan(got, ref):
+    """Bitwise equal (zero signs included); NaN where the reference is NaN."""
+    nan = np.isnan(ref)
+    return (np.array_equal(np.isnan(got), nan)
+            and np.array_equal(got[~nan].view(np.uint32), ref[~nan].view(np.uint32)))
+
+
+def test_hs_jacobi_division_edge_cases(K):
+    """Operands the reciprocal quotient must hand to the IEEE division: zero and
+    negative-zero numerators, denominators outside [2^-24, 2^24] (tiny alpha^2
+    over flat regions, huge gradients), quotients outside [2^-76, 2^76] (huge and
+    subnormal It), overflow to inf and NaN.  Every variant, every iteration count
+    it takes, against the IEEE oracle bit for bit (NaN where the oracle is)."""
+    H, W = 100, 200  # interior (unclipped) tiles for 64x32 and 64x64 windows
+    I1, I2, _, _ = S.hs_pair(H, W, seed=5)
+    Ix, Iy, It = oracle.hs_setup(I1, I2, TAPS["Dx"], TAPS["Dy"], TAPS["Dt"])
+    Ix, Iy, It = Ix.copy(), Iy.copy(), It.copy()
+    u0 = np.zeros((H, W), np.float32)
+    v0 = np.zeros((H, W), np.float32)
+    Ix[:, 40:80] = 0.0                          # flat: den = alpha^2 = 1e-9 (< 2^-24)
+    Iy[:, 40:80] = 0.0
+    It[:, 40:60] = 0.0                          # zero numerators
+    It[:, 60:80] = -0.0                         # negative-zero numerators ...
+    u0[:, 60:80] = -1.0                         # ... with Ix*ubar = -0
+    v0[:, 60:80] = -1.0
+    Ix[:, 80:120] = 3e12                        # den ~ 2e25 (> 2^24)
+    It[:, 120:160] = 1e30                       # quotient > 2^76; products overflow
+    Ix[:, 160:200] = 1.0                        # subnormal numerators
+    Iy[:, 160:200] = 0.0
+    It[:, 160:200] = np.float32(1e-40)
+    It[50, 10] = np.inf                         # inf and NaN spread from one cell
+    It[20, 30] = np.nan
+    a2 = 1e-9
+    for iters in (3, 9):
+        ref_u, ref_v = oracle.hs_jacobi(Ix, Iy, It, a2, TAPS["Avg"], iters, u0, v0)
+        assert np.isnan(ref_u).any() and not np.isnan(ref_u).all()
+        for vi, k in _variants(K):
+            u, v = dev(u0), dev(v0)
+            K.hs_jacobi(dev(Ix), dev(Iy), dev(It), a2, TAPS["Avg"], iters, u, v, variant=vi, k=k)
+            assert _bitwise_equal_nan(host(u), ref_u), (iters, vi, k)
+            assert _bitwise_equal_nan(host(v), ref_v), (iters, vi, k)
+
+
 @pytest.mark.parametrize("n", [1, 33, 4097])
 def test_dia1d_bitexact(K, n):
     """1-D DIA SpMV (scipy dia_matrix layout) against the fp32 oracle."""

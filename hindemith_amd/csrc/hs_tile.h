@@ -119,7 +119,7 @@ struct HsWindow {
 template <int NT, int TW, int TH, int KMAX, int NTAP, bool LOUT = false, bool DB = false,
           bool FDIV = false,
           bool SHARED = !LOUT && ((TW + 2 * KMAX) * (TH + 2 * KMAX) + NT - 1) / NT <= 8,
-          bool PF = SHARED && DB && !FDIV>
+          bool PF = SHARED && DB>
 __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
                                         const HsTapTable<TW + 2 * KMAX> &tp, float2 *uv,
                                         float *lin = nullptr) {
@@ -179,13 +179,15 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
     }
     __syncthreads();
 
-    // One Jacobi step.  PAR (compile time) is the window step it reads with
-    // DB, (it - 1) & 1, so both windows' tap addresses are constants.
+    // One Jacobi step.  PAR (compile time, SHARED builds) is the window step
+    // it reads with DB, (it - 1) & 1, so both windows' tap addresses are
+    // constants; other builds pass 0 and select the window at run time.
     auto step = [&](const int it, auto par_c) {
         constexpr int PAR = decltype(par_c)::value;
         const int act_min = it << 27;
-        const float2 *uvr = DB ? uv + PAR * NPX : uv;
-        float2 *uvw = DB ? uv + (PAR ^ 1) * NPX : uv;
+        const int par = SHARED ? PAR : (it - 1) & 1;
+        const float2 *uvr = DB ? uv + par * NPX : uv;
+        float2 *uvw = DB ? uv + (par ^ 1) * NPX : uv;
         float nu[P], nv[P];
         bool act[P];
         if (PREFETCH && !clipped) {
@@ -193,7 +195,7 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
             // (updated or not: the window is padded) before cell j is
             // computed, so LDS latency overlaps arithmetic instead of stalling
             // the wave; the rest of a cell's taps are read as it starts.
-            constexpr int NPF = PREFETCH ? (NTAP + 1) / 2 : 1;
+            constexpr int NPF = !PREFETCH ? 1 : FDIV ? (NTAP + 3) / 4 : (NTAP + 1) / 2;
             const float2 *b0 = uvr + tid;
             float2 qa[NPF];
 #pragma unroll
@@ -305,7 +307,7 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
         }
         __syncthreads();
     };
-    if (DB) {
+    if (DB && SHARED) {
         for (int it = 1; it <= k; it += 2) {
             step(it, std::integral_constant<int, 0>());
             if (it < k) step(it + 1, std::integral_constant<int, 1>());

@@ -103,7 +103,8 @@ __device__ inline float setup_tap(const float *__restrict__ x, int H, int W, int
 // WPE: waves per SIMD the kernel is compiled to allow (the compiler must fit
 // its VGPRs and SGPRs to it), from which the resident grid is derived.
 // DB: two LDS windows, one barrier per Jacobi step (see hs_tile).
-template <int NT, int TW, int TH, int KMAX, int NTAP, int WPE, bool DB = false>
+template <int NT, int TW, int TH, int KMAX, int NTAP, int WPE, bool DB = false,
+          bool FDIV = false>
 __global__ __launch_bounds__(NT, WPE) void hs_batched_kernel(BatchArgs b, SetupTaps st,
                                                              Taps taps) {
     using Win = HsWindow<NT, TW, TH, KMAX, DB>;
@@ -171,8 +172,9 @@ __global__ __launch_bounds__(NT, WPE) void hs_batched_kernel(BatchArgs b, SetupT
             im.H = P.H;
             im.W = P.W;
             im.tiles_x = P.tiles_x;
-            // the double-buffered builds have registers for shared tap addresses
-            hs_tile<NT, TW, TH, KMAX, NTAP, false, DB, false, DB, false>(im, w - P.tile_base, kk, tp, uv);
+            // per-cell tap addresses: the persistent kernel's own state leaves
+            // no registers for the shared ones
+            hs_tile<NT, TW, TH, KMAX, NTAP, false, DB, FDIV, false, false>(im, w - P.tile_base, kk, tp, uv);
         }
         arrivals += gridDim.x;
         if (!grid_sync(b.barrier, arrivals, b.status, b.spin_limit)) return;
@@ -209,6 +211,10 @@ struct BVariant {
     {NT, TW, TH, K, WPE,                                                                  \
      {hs_batched_kernel<NT, TW, TH, K, 8, WPE, true>,                                     \
       hs_batched_kernel<NT, TW, TH, K, 0, WPE, true>}}
+#define HB_F(NT, TW, TH, K, WPE)                                                          \
+    {NT, TW, TH, K, WPE,                                                                  \
+     {hs_batched_kernel<NT, TW, TH, K, 8, WPE, true, true>,                               \
+      hs_batched_kernel<NT, TW, TH, K, 0, WPE, true, true>}}
 static const BVariant kBVariants[] = {
     HB_V(1024, 64, 32, 4, 8),   // 0: two work-groups per CU
     HB_V(1024, 64, 32, 8, 8),   // 1: two work-groups per CU
@@ -217,9 +223,11 @@ static const BVariant kBVariants[] = {
     HB_V(1024, 64, 64, 8, 4),   // 4: one
     HB_D(1024, 64, 32, 4, 8),   // 5: double-buffered window, two per CU
     HB_D(1024, 64, 32, 8, 8),   // 6: double-buffered window, two per CU
+    HB_F(1024, 64, 32, 4, 8),   // 7: 5 with the reciprocal quotient (k = 8 would spill)
 };
 #undef HB_V
 #undef HB_D
+#undef HB_F
 static constexpr int kNumBVariants = sizeof(kBVariants) / sizeof(kBVariants[0]);
 
 // A cached launch: device descriptors, barrier words and (optionally) the

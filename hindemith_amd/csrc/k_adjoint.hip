@@ -24,6 +24,7 @@
 // step needs on the pixel itself -- t and (ubar, vbar) -- then walks the
 // segment backwards.
 #include "hmx_common.h"
+#include "hs_tile.h"
 
 #include <algorithm>
 #include <cstring>
@@ -89,16 +90,18 @@ struct AdjTbArgs {
     const float *Ix, *Iy, *It;
     const float *gu_in, *gv_in;  // dL/dx_hi
     float *gu_out, *gv_out;      // dL/dx_{hi-m}
-    // step j (1..m) reverses forward step hi-j, linearised about lt/lub/lvb[j-1]
-    const float *lt[8], *lub[8], *lvb[8];
+    // step j (1..m) reverses forward step hi-j, linearised about (t, ubar,
+    // vbar) at lin_hi - (j-1)*lin_step, + plane, + 2*plane
+    const float *lin_hi;
+    int64_t lin_step, plane;
     float *gIx, *gIy, *gIt;
-    double *ga2;                 // [pair][tile] running sums of -s t
+    float *ga2;                  // per pixel: running sum over steps of -s t
     const float *alpha2_b;
     int H, W, m, tiles_x, ntiles;
     int64_t bstride;
 };
 
-constexpr int ADJ_TW = 64, ADJ_TH = 32, ADJ_NT = 1024, ADJ_MMAX = 4;
+constexpr int ADJ_TW = 64, ADJ_TH = 32, ADJ_NT = 1024, ADJ_MMAX = 8;
 
 // Compiled for 8 waves per SIMD: two 1024-thread work-groups per CU.
 // Thread layout: slots 0..PT-1 of a thread are tile pixels in tile order
@@ -118,7 +121,6 @@ __global__ __launch_bounds__(NT, 8) void hs_adj_tb_kernel(AdjTbArgs a, Taps taps
     constexpr int PH = (NPX - TP + NT - 1) / NT;
     constexpr int P = PT + PH;
     __shared__ float2 GB[2][NPX];  // (g_ubar, g_vbar) on the current region
-    __shared__ double red[2][NT / 64];
 
     const int b = blockIdx.y;
     const int64_t off = (int64_t)b * a.bstride;
@@ -135,13 +137,16 @@ __global__ __launch_bounds__(NT, 8) void hs_adj_tb_kernel(AdjTbArgs a, Taps taps
     const int tw = x1 - x0, th = y1 - y0;
     const int tx = x0 - gx0, ty = y0 - gy0;  // tile origin in region coordinates
 
-    // region coordinates (ry << 16 | rx) of every slot, -1 if none
+    // Every slot's region coordinates and its distance D outside the tile,
+    // packed D << 27 | ry << 16 | rx (-1: no pixel).  The region of reach e
+    // (the tile grown by e, clipped to the image) holds the slot exactly
+    // when D <= e, i.e. when (unsigned)pk < (e + 1) << 27: one compare.
     int pk[P];
 #pragma unroll
     for (int j = 0; j < PT; ++j) {
         const int i = threadIdx.x + j * NT;
         const int ly = i / TW, lx = i - ly * TW;
-        pk[j] = (lx < tw && ly < th) ? ((ty + ly) << 16) | (tx + lx) : -1;
+        pk[j] = (lx < tw && ly < th) ? ((ty + ly) << 16) | (tx + lx) : -1;  // D = 0
     }
     {
         const int lw = tx, rw = gx1 - x1, sw = lw + rw;
@@ -163,22 +168,31 @@ __global__ __launch_bounds__(NT, 8) void hs_adj_tb_kernel(AdjTbArgs a, Taps taps
                 ry = ty + r;
                 rx = c < lw ? c : tx + tw + (c - lw);
             }
-            pk[PT + j] = rx < 0 ? -1 : (ry << 16) | rx;
+            if (rx < 0) {
+                pk[PT + j] = -1;
+            } else {
+                const int gx = gx0 + rx, gy = gy0 + ry;
+                const int dx = max(max(x0 - gx, gx - x1 + 1), 0);
+                const int dy = max(max(y0 - gy, gy - y1 + 1), 0);
+                pk[PT + j] = (max(dx, dy) << 27) | (ry << 16) | rx;
+            }
         }
     }
+    auto in_reach = [](int pkj, int e) { return (unsigned)pkj < ((unsigned)e + 1u) << 27; };
+    const HsTapTable<PITCH> tp(taps);
 
     const float *Ix = a.Ix + off, *Iy = a.Iy + off;
     float cx[P], cy[P], cd[P];
     float2 g[P];
-    float ax[PT], ay[PT], at[PT];
+    float ax[PT], ay[PT], at[PT], aa[PT];
 #pragma unroll
     for (int j = 0; j < P; ++j) {
         cx[j] = cy[j] = 0.0f;
         cd[j] = 1.0f;
         g[j] = make_float2(0.0f, 0.0f);
-        if (j < PT) ax[j] = ay[j] = at[j] = 0.0f;
+        if (j < PT) ax[j] = ay[j] = at[j] = aa[j] = 0.0f;
         if (pk[j] < 0) continue;
-        const int rx = pk[j] & 0xffff, ry = pk[j] >> 16;
+        const int rx = pk[j] & 0xffff, ry = (pk[j] >> 16) & 0x7ff;
         const int64_t q = (int64_t)(gy0 + ry) * W + gx0 + rx;
         const float ix = Ix[q], iy = Iy[q];
         cx[j] = ix;
@@ -192,23 +206,23 @@ __global__ __launch_bounds__(NT, 8) void hs_adj_tb_kernel(AdjTbArgs a, Taps taps
             ax[j] = a.gIx[off + q];
             ay[j] = a.gIy[off + q];
             at[j] = a.gIt[off + q];
+            aa[j] = a.ga2[off + q];
         }
     }
-    double tile_a2 = 0.0;  // running per-tile sum, held by thread 0
-    if (threadIdx.x == 0) tile_a2 = a.ga2[(int64_t)b * a.ntiles + t];
 
     // (t, ubar, vbar) of forward step hi - step at the tile slots
     float lt[PT], lu[PT], lv[PT];
     auto load_lin = [&](int step) {
-        const float *T0 = a.lt[step - 1] + off, *U0 = a.lub[step - 1] + off,
-                    *V0 = a.lvb[step - 1] + off;
+        const float *T0 = a.lin_hi - (int64_t)(step - 1) * a.lin_step + off;
+        const float *U0 = T0 + a.plane, *V0 = T0 + 2 * a.plane;
 #pragma unroll
         for (int j = 0; j < PT; ++j) {
             int pkj = pk[j];
             asm volatile("" : "+v"(pkj));  // recompute the offsets per step
             lt[j] = lu[j] = lv[j] = 0.0f;
             if (pkj >= 0) {
-                const int64_t q = (int64_t)(gy0 + (pkj >> 16)) * W + gx0 + (pkj & 0xffff);
+                const int64_t q =
+                    (int64_t)(gy0 + ((pkj >> 16) & 0x7ff)) * W + gx0 + (pkj & 0xffff);
                 lt[j] = T0[q];
                 lu[j] = U0[q];
                 lv[j] = V0[q];
@@ -216,68 +230,63 @@ __global__ __launch_bounds__(NT, 8) void hs_adj_tb_kernel(AdjTbArgs a, Taps taps
         }
     };
 
-    // pass A of step `step` on R_{m-step+1}: s, (g_ubar, g_vbar) into GB[buf],
-    // the tile's accumulator terms, and the step's part of the alpha sum
+    // pass A of step `step` on R_{m-step+1}: s, (g_ubar, g_vbar) into GB[buf]
+    // and the tile's accumulator terms (the alpha term -s t included)
     auto pass_a = [&](int step, int buf) {
         const int e = m - step + 1;
-        const int rx0 = max(x0 - e, 0) - gx0, ry0 = max(y0 - e, 0) - gy0;
-        const int rx1 = min(x1 + e, W) - gx0, ry1 = min(y1 + e, H) - gy0;
-        double a2 = 0.0;
 #pragma unroll
         for (int j = 0; j < P; ++j) {
             int pkj = pk[j];
             asm volatile("" : "+v"(pkj));
-            const int rx = pkj & 0xffff, ry = pkj >> 16;
-            if (pkj < 0 || rx < rx0 || rx >= rx1 || ry < ry0 || ry >= ry1) continue;
-            const float s = -(cx[j] * g[j].x + cy[j] * g[j].y) * cd[j];
-            GB[buf][ry * PITCH + rx] = make_float2(g[j].x + cx[j] * s, g[j].y + cy[j] * s);
+            if (!in_reach(pkj, e)) continue;
+            const int rx = pkj & 0xffff, ry = (pkj >> 16) & 0x7ff;
+            // (fused multiply-adds: the adjoint is checked against fp64 and
+            // finite differences, not bit for bit)
+            const float s = -fmaf(cx[j], g[j].x, cy[j] * g[j].y) * cd[j];
+            GB[buf][ry * PITCH + rx] = make_float2(fmaf(cx[j], s, g[j].x), fmaf(cy[j], s, g[j].y));
             if (j < PT) {  // a tile pixel (always inside the region)
                 const float tt = lt[j], st2 = 2.0f * s * tt;
-                ax[j] += (-g[j].x * tt + s * lu[j]) - st2 * cx[j];
-                ay[j] += (-g[j].y * tt + s * lv[j]) - st2 * cy[j];
+                ax[j] = fmaf(-st2, cx[j], fmaf(s, lu[j], fmaf(-g[j].x, tt, ax[j])));
+                ay[j] = fmaf(-st2, cy[j], fmaf(s, lv[j], fmaf(-g[j].y, tt, ay[j])));
                 at[j] += s;
-                a2 += (double)(-s * tt);
+                aa[j] = fmaf(-s, tt, aa[j]);
             }
-        }
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) a2 += __shfl_xor(a2, o, 64);
-        if ((threadIdx.x & 63) == 0) red[step & 1][threadIdx.x >> 6] = a2;
-    };
-    // thread 0 adds the step's per-wave sums, in wave order, after the barrier
-    auto add_a2 = [&](int step) {
-        if (threadIdx.x == 0) {
-            double r = 0.0;
-            for (int i = 0; i < NT / 64; ++i) r += red[step & 1][i];
-            tile_a2 += r;
         }
     };
 
     load_lin(1);
     pass_a(1, 0);
     __syncthreads();
-    add_a2(1);
 
     for (int step = 1; step <= m; ++step) {
         const int buf = (step - 1) & 1;
+        // the next step's (t, ubar, vbar) are read before the gather, so
+        // their latency overlaps it (pass A of this step has used the
+        // current ones)
+        if (step < m) load_lin(step + 1);
         // pass B on R_{m-step}: g <- A^T g_b (gather over sources that clamp onto p)
         const int e = m - step;
-        const int rx0 = max(x0 - e, 0) - gx0, ry0 = max(y0 - e, 0) - gy0;
-        const int rx1 = min(x1 + e, W) - gx0, ry1 = min(y1 + e, H) - gy0;
         const float2 *GBb = GB[buf];
 #pragma unroll
         for (int j = 0; j < P; ++j) {
             int pkj = pk[j];
             asm volatile("" : "+v"(pkj));
-            const int rx = pkj & 0xffff, ry = pkj >> 16;
-            if (pkj < 0 || rx < rx0 || rx >= rx1 || ry < ry0 || ry >= ry1) continue;
+            if (!in_reach(pkj, e)) continue;
+            const int rx = pkj & 0xffff, ry = (pkj >> 16) & 0x7ff;
             const int gx = gx0 + rx, gy = gy0 + ry;
             float au = 0.0f, av = 0.0f;
             if (!clipped || (gx > 0 && gx < W - 1 && gy > 0 && gy < H - 1)) {
                 const float2 *gb = GBb + ry * PITCH + rx;
-                for (int d = 0; d < taps.nd; ++d) {
-                    const float2 q = gb[-(taps.oy[d] * PITCH + taps.ox[d])];
-                    au = fmaf(taps.w[d], q.x, au);
-                    av = fmaf(taps.w[d], q.y, av);
+                // unrolled over the largest tap count, each tap tested: a
+                // straight-line 8-tap gather would hold too many reads in
+                // flight for the registers of 8 waves per SIMD
+#pragma unroll
+                for (int d = 0; d < HS_MAX_TAPS; ++d) {
+                    if (d < tp.nd) {
+                        const float2 q = gb[-tp.dl[d]];
+                        au = fmaf(tp.w[d], q.x, au);
+                        av = fmaf(tp.w[d], q.y, av);
+                    }
                 }
             } else {
                 for (int d = 0; d < taps.nd; ++d) {
@@ -295,29 +304,25 @@ __global__ __launch_bounds__(NT, 8) void hs_adj_tb_kernel(AdjTbArgs a, Taps taps
             g[j] = make_float2(au, av);
         }
         if (step < m) {
-            // (loaded here rather than during pass B: holding them across the
-            // gather would spill registers)
-            load_lin(step + 1);
             // GB[buf ^ 1] was last read in pass B of step - 1, before the
             // previous barrier
             pass_a(step + 1, buf ^ 1);
             __syncthreads();
-            add_a2(step + 1);
         }
     }
 
 #pragma unroll
     for (int j = 0; j < PT; ++j) {
         if (pk[j] < 0) continue;
-        const int rx = pk[j] & 0xffff, ry = pk[j] >> 16;
+        const int rx = pk[j] & 0xffff, ry = (pk[j] >> 16) & 0x7ff;
         const int64_t q = off + (int64_t)(gy0 + ry) * W + gx0 + rx;
         a.gu_out[q] = g[j].x;
         a.gv_out[q] = g[j].y;
         a.gIx[q] = ax[j];
         a.gIy[q] = ay[j];
         a.gIt[q] = at[j];
+        a.ga2[q] = aa[j];
     }
-    if (threadIdx.x == 0) a.ga2[(int64_t)b * a.ntiles + t] = tile_a2;
 }
 
 // dL/dI1, dL/dI2 from the derivative-cube accumulators.
@@ -346,12 +351,37 @@ __global__ void alpha_sq_kernel(const float *alpha, float *alpha2, int B) {
     if (b < B) alpha2[b] = alpha[b] * alpha[b];
 }
 
-__global__ void galpha_kernel(const double *__restrict__ ga2, int ntiles, const float *alpha,
-                              float *galpha, int B) {
-    const int b = blockIdx.x;
-    if (b >= B || threadIdx.x != 0) return;
+// dL/dalpha = 2 alpha times the sum of the per-pixel alpha terms, summed in
+// fp64 in a fixed order so the result does not depend on timing: block
+// (c, b) reduces chunk c of pair b (thread i takes elements i, i + 256, ...
+// of the chunk, then a fixed tree), and galpha_finish adds pair b's
+// GALPHA_CHUNKS partials in chunk order.
+constexpr int GALPHA_CHUNKS = 64;
+
+__global__ __launch_bounds__(256) void galpha_partial_kernel(const float *__restrict__ ga2,
+                                                             int64_t n, double *partial) {
+    __shared__ double part[256];
+    const int c = blockIdx.x, b = blockIdx.y;
+    const int64_t per = (n + GALPHA_CHUNKS - 1) / GALPHA_CHUNKS;
+    const int64_t lo = c * per, hi = lo + per < n ? lo + per : n;
+    const float *g = ga2 + (int64_t)b * n;
     double s = 0.0;
-    for (int i = 0; i < ntiles; ++i) s += ga2[(int64_t)b * ntiles + i];
+    for (int64_t i = lo + threadIdx.x; i < hi; i += 256) s += (double)g[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(int64_t)b * GALPHA_CHUNKS + c] = part[0];
+}
+
+__global__ void galpha_finish_kernel(const double *__restrict__ partial, const float *alpha,
+                                     float *galpha, int B) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    double s = 0.0;
+    for (int c = 0; c < GALPHA_CHUNKS; ++c) s += partial[(int64_t)b * GALPHA_CHUNKS + c];
     galpha[b] = (float)(2.0 * (double)alpha[b] * s);
 }
 
@@ -412,10 +442,10 @@ extern "C" int hmx_hs_ckpt_count(int iters, int ckpt_every) {
 extern "C" int64_t hmx_hs_adjoint_workspace(int B, int H, int W, int ckpt_every) {
     const int64_t n = (int64_t)B * H * W;
     // Ix Iy It | gIx gIy gIt | 2 gradient ping-pong pairs | 2 recompute state
-    // ping-pong pairs | (t, ubar, vbar) of every step of one segment
-    const int64_t floats = 3 * n + 3 * n + 4 * n + 4 * n + 3 * n * std::max(ckpt_every, 1);
-    const int64_t doubles = (int64_t)B * adj_tiles(H, W);
-    return floats * 4 + doubles * 8 + (int64_t)B * 4 + 256;
+    // ping-pong pairs | per-pixel alpha terms | (t, ubar, vbar) of every step
+    // of one segment | alpha^2 per pair | fp64 partial sums of the alpha terms
+    const int64_t floats = 3 * n + 3 * n + 4 * n + 4 * n + n + 3 * n * std::max(ckpt_every, 1);
+    return floats * 4 + (int64_t)B * 4 + 256 + (int64_t)B * GALPHA_CHUNKS * 8;
 }
 
 extern "C" int64_t hmx_hs_forward_workspace(int B, int H, int W) {
@@ -490,15 +520,16 @@ extern "C" int hmx_hs_adjoint(const float *I1, const float *I2, int B, int H, in
     float *gIx = ws + 3 * n, *gIy = ws + 4 * n, *gIt = ws + 5 * n;
     float *g[2][2] = {{ws + 6 * n, ws + 7 * n}, {ws + 8 * n, ws + 9 * n}};
     float *rs[2][2] = {{ws + 10 * n, ws + 11 * n}, {ws + 12 * n, ws + 13 * n}};
-    float *lin = ws + 14 * n;  // slot j: t, ubar, vbar of the segment's step j
-    double *ga2 = (double *)(((uintptr_t)(lin + 3 * n * ckpt_every) + 255) & ~(uintptr_t)255);
-    float *alpha2 = (float *)(ga2 + (int64_t)B * ntiles);
+    float *ga2 = ws + 14 * n;  // per-pixel running sums of the alpha term
+    float *lin = ws + 15 * n;  // slot j: t, ubar, vbar of the segment's step j
+    float *alpha2 = lin + 3 * n * ckpt_every;
+    double *partial = (double *)(((uintptr_t)(alpha2 + B) + 255) & ~(uintptr_t)255);
 
     hipLaunchKernelGGL(alpha_sq_kernel, dim3((B + 63) / 64), dim3(64), 0, s, alpha, alpha2, B);
     HMX_LAUNCH_CHECK("alpha_sq");
     rc = setup_batch(I1, I2, B, H, W, T.dx, T.dy, T.dt, Ix, Iy, It, s);
     if (!rc) rc = launch_fill_f32(gIx, 0.0f, 3 * n, s);
-    if (!rc) rc = launch_fill_u32((unsigned *)ga2, 0u, 2 * (int64_t)B * ntiles, s);
+    if (!rc) rc = launch_fill_f32(ga2, 0.0f, n, s);
     if (rc) return rc;
     HMX_HIP_TRY(hipMemcpyAsync(g[0][0], gu_N, n * 4, hipMemcpyDeviceToDevice, s));
     HMX_HIP_TRY(hipMemcpyAsync(g[0][1], gv_N, n * 4, hipMemcpyDeviceToDevice, s));
@@ -543,12 +574,9 @@ extern "C" int hmx_hs_adjoint(const float *I1, const float *I2, int B, int H, in
         for (int hi = len; hi > 0;) {
             const int mm = std::min(ADJ_MMAX, hi);
             a.m = mm;
-            for (int st = 1; st <= mm; ++st) {
-                const float *lt = lin + (int64_t)3 * (hi - st) * n;
-                a.lt[st - 1] = lt;
-                a.lub[st - 1] = lt + n;
-                a.lvb[st - 1] = lt + 2 * n;
-            }
+            a.lin_hi = lin + (int64_t)3 * (hi - 1) * n;
+            a.lin_step = (int64_t)3 * n;
+            a.plane = n;
             a.gu_in = g[cur][0];
             a.gv_in = g[cur][1];
             a.gu_out = g[cur ^ 1][0];
@@ -566,8 +594,12 @@ extern "C" int hmx_hs_adjoint(const float *I1, const float *I2, int B, int H, in
     hipLaunchKernelGGL(hs_setup_T_kernel, grid, dim3(256), 0, s, gIx, gIy, gIt, H, W,
                        (int64_t)H * W, T.dx, T.dy, T.dt, gI1, gI2);
     HMX_LAUNCH_CHECK("hs_setup_T");
-    hipLaunchKernelGGL(galpha_kernel, dim3(B), dim3(64), 0, s, ga2, (int)ntiles, alpha, galpha, B);
-    HMX_LAUNCH_CHECK("galpha");
+    hipLaunchKernelGGL(galpha_partial_kernel, dim3(GALPHA_CHUNKS, B), dim3(256), 0, s, ga2,
+                       (int64_t)H * W, partial);
+    HMX_LAUNCH_CHECK("galpha_partial");
+    hipLaunchKernelGGL(galpha_finish_kernel, dim3((B + 63) / 64), dim3(64), 0, s, partial, alpha,
+                       galpha, B);
+    HMX_LAUNCH_CHECK("galpha_finish");
     return HMX_OK;
 }
 

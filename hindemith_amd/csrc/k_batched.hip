@@ -1,5 +1,10 @@
-// a14: many Horn-Schunck pairs of uneven sizes in one persistent launch
-// (no counterpart in the reference, which runs one program call at a time).
+// a14: many Horn-Schunck pairs of uneven sizes in one captured launch
+// sequence (no counterpart in the reference, which runs one program call at a
+// time).  Two forms share the tile list and the tile body; the autotuner
+// picks one per size signature:
+//   * stepped (variants 8-11, the default): one launch per phase over every
+//     tile of every pair (see hs_multi_phase_kernel);
+//   * persistent (variants 0-7), below.
 //
 // The tiles of all pairs form one list.  A grid no larger than what the GPU
 // holds at once walks that list: first the derivative setup of every tile,
@@ -100,6 +105,64 @@ __device__ inline float setup_tap(const float *__restrict__ x, int H, int W, int
     return acc;
 }
 
+// Derivative setup of tile w of the list (Ix = Dx*I1 + Dx*I2, ...), one
+// rounding per op.
+template <int NT, int TW, int TH>
+__device__ __forceinline__ void setup_tile(const BatchArgs &b, const SetupTaps &st, int w) {
+    const PairDesc &P = b.pairs[b.tile_pair[w]];
+    const int t = w - P.tile_base;
+    const int tyi = t / P.tiles_x, txi = t - tyi * P.tiles_x;
+    for (int i = threadIdx.x; i < TW * TH; i += NT) {
+        const int r = tyi * TH + i / TW, c = txi * TW + i % TW;
+        if (r >= P.H || c >= P.W) continue;
+        const int64_t p = (int64_t)r * P.W + c;
+        const float a1 = setup_tap(P.I1, P.H, P.W, r, c, st.dx);
+        const float a2 = setup_tap(P.I2, P.H, P.W, r, c, st.dx);
+        P.Ix[p] = a1 + a2;
+        const float b1 = setup_tap(P.I1, P.H, P.W, r, c, st.dy);
+        const float b2 = setup_tap(P.I2, P.H, P.W, r, c, st.dy);
+        P.Iy[p] = b1 + b2;
+        const float t2 = setup_tap(P.I2, P.H, P.W, r, c, st.dt);
+        const float t1 = setup_tap(P.I1, P.H, P.W, r, c, st.dt);
+        P.It[p] = t2 - t1;
+    }
+}
+
+// (u, v) <- (wu, wv) on tile w (after an odd number of phases).
+template <int NT, int TW, int TH>
+__device__ __forceinline__ void copy_tile(const BatchArgs &b, int w) {
+    const PairDesc &P = b.pairs[b.tile_pair[w]];
+    const int t = w - P.tile_base;
+    const int tyi = t / P.tiles_x, txi = t - tyi * P.tiles_x;
+    for (int i = threadIdx.x; i < TW * TH; i += NT) {
+        const int r = tyi * TH + i / TW, c = txi * TW + i % TW;
+        if (r >= P.H || c >= P.W) continue;
+        const int64_t p = (int64_t)r * P.W + c;
+        P.u[p] = P.wu[p];
+        P.v[p] = P.wv[p];
+    }
+}
+
+// The image a phase reads and writes: even phases (u, v) -> (wu, wv), odd
+// phases back.
+__device__ __forceinline__ HsImage phase_image(const PairDesc &P, int ph) {
+    HsImage im;
+    im.Ix = P.Ix;
+    im.Iy = P.Iy;
+    im.It = P.It;
+    im.den = nullptr;
+    const bool even = (ph & 1) == 0;
+    im.u_in = even ? P.u : P.wu;
+    im.v_in = even ? P.v : P.wv;
+    im.u_out = even ? P.wu : P.u;
+    im.v_out = even ? P.wv : P.v;
+    im.alpha2 = P.alpha2;
+    im.H = P.H;
+    im.W = P.W;
+    im.tiles_x = P.tiles_x;
+    return im;
+}
+
 // WPE: waves per SIMD the kernel is compiled to allow (the compiler must fit
 // its VGPRs and SGPRs to it), from which the resident grid is derived.
 // DB: two LDS windows, one barrier per Jacobi step (see hs_tile).
@@ -118,27 +181,8 @@ __global__ __launch_bounds__(NT, WPE) void hs_batched_kernel(BatchArgs b, SetupT
     int base, cnt;
     xcd_share(b.total_tiles, xcd, base, cnt);
 
-    // derivative setup of every tile (Ix = Dx*I1 + Dx*I2, ...), one rounding per op
-    for (int i = slot; i < cnt; i += per_xcd) {
-        const int w = base + i;
-        const PairDesc &P = b.pairs[b.tile_pair[w]];
-        const int t = w - P.tile_base;
-        const int tyi = t / P.tiles_x, txi = t - tyi * P.tiles_x;
-        for (int i = threadIdx.x; i < TW * TH; i += NT) {
-            const int r = tyi * TH + i / TW, c = txi * TW + i % TW;
-            if (r >= P.H || c >= P.W) continue;
-            const int64_t p = (int64_t)r * P.W + c;
-            const float a1 = setup_tap(P.I1, P.H, P.W, r, c, st.dx);
-            const float a2 = setup_tap(P.I2, P.H, P.W, r, c, st.dx);
-            P.Ix[p] = a1 + a2;
-            const float b1 = setup_tap(P.I1, P.H, P.W, r, c, st.dy);
-            const float b2 = setup_tap(P.I2, P.H, P.W, r, c, st.dy);
-            P.Iy[p] = b1 + b2;
-            const float t2 = setup_tap(P.I2, P.H, P.W, r, c, st.dt);
-            const float t1 = setup_tap(P.I1, P.H, P.W, r, c, st.dt);
-            P.It[p] = t2 - t1;
-        }
-    }
+    // derivative setup of every tile
+    for (int i = slot; i < cnt; i += per_xcd) setup_tile<NT, TW, TH>(b, st, base + i);
     arrivals += gridDim.x;
     if (!grid_sync(b.barrier, arrivals, b.status, b.spin_limit)) return;
 
@@ -180,41 +224,74 @@ __global__ __launch_bounds__(NT, WPE) void hs_batched_kernel(BatchArgs b, SetupT
         if (!grid_sync(b.barrier, arrivals, b.status, b.spin_limit)) return;
     }
 
-    if (nph & 1) {  // the last phase wrote (wu, wv)
-        for (int i = slot; i < cnt; i += per_xcd) {
-            const int w = base + i;
-            const PairDesc &P = b.pairs[b.tile_pair[w]];
-            const int t = w - P.tile_base;
-            const int tyi = t / P.tiles_x, txi = t - tyi * P.tiles_x;
-            for (int i = threadIdx.x; i < TW * TH; i += NT) {
-                const int r = tyi * TH + i / TW, c = txi * TW + i % TW;
-                if (r >= P.H || c >= P.W) continue;
-                const int64_t p = (int64_t)r * P.W + c;
-                P.u[p] = P.wu[p];
-                P.v[p] = P.wv[p];
-            }
-        }
-    }
+    if (nph & 1)  // the last phase wrote (wu, wv)
+        for (int i = slot; i < cnt; i += per_xcd) copy_tile<NT, TW, TH>(b, base + i);
 }
 
 // ---------------------------------------------------------------------------
 
+
+// ---------------------------------------------------------------------------
+// Stepped form: the same tile list, one ordinary launch per phase (plus one
+// for the setup and, after an odd number of phases, one copy).  Kernel
+// boundaries order the phases, so there is no grid barrier and no residency
+// limit, and the tile body is the single-image kernel's full form (shared tap
+// addresses, read-ahead, reciprocal quotient).  One work-group per tile; the
+// grid is the whole list, dealt so each XCD gets a contiguous range.
+
+template <int NT, int TW, int TH>
+__global__ __launch_bounds__(NT) void hs_multi_setup_kernel(BatchArgs b, SetupTaps st) {
+    setup_tile<NT, TW, TH>(b, st, xcd_remap(blockIdx.x, b.total_tiles));
+}
+
+template <int NT, int TW, int TH, int KMAX, int NTAP, bool DB, bool FDIV>
+__global__ __launch_bounds__(NT) void hs_multi_phase_kernel(BatchArgs b, Taps taps, int ph,
+                                                            int kk) {
+    using Win = HsWindow<NT, TW, TH, KMAX, DB>;
+    __shared__ float2 uvm[Win::SIZE];
+    const HsTapTable<TW + 2 * KMAX> tp(taps);
+    const int w = xcd_remap(blockIdx.x, b.total_tiles);
+    const PairDesc &P = b.pairs[b.tile_pair[w]];
+    const HsImage im = phase_image(P, ph);
+    hs_tile<NT, TW, TH, KMAX, NTAP, false, DB, FDIV>(im, w - P.tile_base, kk, tp,
+                                                      uvm + Win::PAD);
+}
+
+template <int NT, int TW, int TH>
+__global__ __launch_bounds__(NT) void hs_multi_copy_kernel(BatchArgs b) {
+    copy_tile<NT, TW, TH>(b, xcd_remap(blockIdx.x, b.total_tiles));
+}
+
+// ---------------------------------------------------------------------------
 struct BVariant {
     int nt, tw, th, kmax, wpe;
-    void (*kernel[2])(BatchArgs, SetupTaps, Taps);  // 8 taps, any count
+    void (*kernel[2])(BatchArgs, SetupTaps, Taps);  // persistent: 8 taps, any count
+    // stepped form (kernel[] null): setup, phase (8 taps, any count), copy
+    void (*setup)(BatchArgs, SetupTaps);
+    void (*phase[2])(BatchArgs, Taps, int, int);
+    void (*copy)(BatchArgs);
 };
 
 #define HB_V(NT, TW, TH, K, WPE)                                                          \
     {NT, TW, TH, K, WPE,                                                                  \
-     {hs_batched_kernel<NT, TW, TH, K, 8, WPE>, hs_batched_kernel<NT, TW, TH, K, 0, WPE>}}
+     {hs_batched_kernel<NT, TW, TH, K, 8, WPE>, hs_batched_kernel<NT, TW, TH, K, 0, WPE>}, \
+     nullptr, {nullptr, nullptr}, nullptr}
 #define HB_D(NT, TW, TH, K, WPE)                                                          \
     {NT, TW, TH, K, WPE,                                                                  \
      {hs_batched_kernel<NT, TW, TH, K, 8, WPE, true>,                                     \
-      hs_batched_kernel<NT, TW, TH, K, 0, WPE, true>}}
+      hs_batched_kernel<NT, TW, TH, K, 0, WPE, true>},                                    \
+     nullptr, {nullptr, nullptr}, nullptr}
 #define HB_F(NT, TW, TH, K, WPE)                                                          \
     {NT, TW, TH, K, WPE,                                                                  \
      {hs_batched_kernel<NT, TW, TH, K, 8, WPE, true, true>,                               \
-      hs_batched_kernel<NT, TW, TH, K, 0, WPE, true, true>}}
+      hs_batched_kernel<NT, TW, TH, K, 0, WPE, true, true>},                              \
+     nullptr, {nullptr, nullptr}, nullptr}
+// stepped form (DB: double-buffered window, FDIV: reciprocal quotient)
+#define HB_S(NT, TW, TH, K, DB, FDIV)                                                     \
+    {NT, TW, TH, K, 0, {nullptr, nullptr}, hs_multi_setup_kernel<NT, TW, TH>,            \
+     {hs_multi_phase_kernel<NT, TW, TH, K, 8, DB, FDIV>,                                  \
+      hs_multi_phase_kernel<NT, TW, TH, K, 0, DB, FDIV>},                                 \
+     hs_multi_copy_kernel<NT, TW, TH>}
 static const BVariant kBVariants[] = {
     HB_V(1024, 64, 32, 4, 8),   // 0: two work-groups per CU
     HB_V(1024, 64, 32, 8, 8),   // 1: two work-groups per CU
@@ -224,10 +301,15 @@ static const BVariant kBVariants[] = {
     HB_D(1024, 64, 32, 4, 8),   // 5: double-buffered window, two per CU
     HB_D(1024, 64, 32, 8, 8),   // 6: double-buffered window, two per CU
     HB_F(1024, 64, 32, 4, 8),   // 7: 5 with the reciprocal quotient (k = 8 would spill)
+    HB_S(1024, 64, 32, 8, true, true),   // 8: stepped, as single-image variant 24
+    HB_S(1024, 64, 32, 4, true, true),   // 9: stepped, as variant 25
+    HB_S(1024, 64, 32, 8, true, false),  // 10: stepped, as variant 17
+    HB_S(512, 64, 32, 4, true, true),    // 11: stepped, as variant 26
 };
 #undef HB_V
 #undef HB_D
 #undef HB_F
+#undef HB_S
 static constexpr int kNumBVariants = sizeof(kBVariants) / sizeof(kBVariants[0]);
 
 // A cached launch: device descriptors, barrier words and (optionally) the
@@ -278,8 +360,25 @@ static int launch_batch(const BatchPlan &P, const SetupTaps &st, const Taps &av,
     a.iters = iters;
     a.k = P.k;
     const BVariant &V = kBVariants[P.variant];
-    hipLaunchKernelGGL(V.kernel[av.nd == 8 ? 0 : 1], dim3(P.grid), dim3(V.nt), 0, s, a, st, av);
-    HMX_LAUNCH_CHECK("hs_batched");
+    if (V.kernel[0]) {
+        hipLaunchKernelGGL(V.kernel[av.nd == 8 ? 0 : 1], dim3(P.grid), dim3(V.nt), 0, s, a, st,
+                           av);
+        HMX_LAUNCH_CHECK("hs_batched");
+        return HMX_OK;
+    }
+    const dim3 grid(P.total_tiles), block(V.nt);
+    hipLaunchKernelGGL(V.setup, grid, block, 0, s, a, st);
+    HMX_LAUNCH_CHECK("hs_multi_setup");
+    const int nph = (iters + P.k - 1) / P.k;
+    for (int ph = 0; ph < nph; ++ph) {
+        const int kk = std::min(P.k, iters - ph * P.k);
+        hipLaunchKernelGGL(V.phase[av.nd == 8 ? 0 : 1], grid, block, 0, s, a, av, ph, kk);
+        HMX_LAUNCH_CHECK("hs_multi_phase");
+    }
+    if (nph & 1) {
+        hipLaunchKernelGGL(V.copy, grid, block, 0, s, a);
+        HMX_LAUNCH_CHECK("hs_multi_copy");
+    }
     return HMX_OK;
 }
 
@@ -290,8 +389,14 @@ static int launch_batch(const BatchPlan &P, const SetupTaps &st, const Taps &av,
 // be resident at once, or the grid barrier waits until its spin limit.
 static int resident_per_cu(int variant, int nd, int device, int &api, int &own, int &cus) {
     const BVariant &V = kBVariants[variant];
-    const void *kern = reinterpret_cast<const void *>(V.kernel[nd == 8 ? 0 : 1]);
+    const void *kern = V.kernel[0] ? reinterpret_cast<const void *>(V.kernel[nd == 8 ? 0 : 1])
+                                   : reinterpret_cast<const void *>(V.phase[nd == 8 ? 0 : 1]);
     HMX_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&api, kern, V.nt, 0));
+    if (!V.kernel[0]) {  // stepped form: no residency requirement
+        HMX_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+        own = api;
+        return HMX_OK;
+    }
     hipFuncAttributes fa;
     HMX_HIP_TRY(hipFuncGetAttributes(&fa, kern));
     int lds = 0;
@@ -311,6 +416,10 @@ static int resident_per_cu(int variant, int nd, int device, int &api, int &own, 
 }
 
 static int resident_grid(int variant, int nd, int total_tiles, int device, int &grid) {
+    if (!kBVariants[variant].kernel[0]) {  // stepped form: one work-group per tile
+        grid = total_tiles;
+        return HMX_OK;
+    }
     int api = 0, own = 0, cus = 0;
     int rc = resident_per_cu(variant, nd, device, api, own, cus);
     if (rc) return rc;
@@ -359,7 +468,7 @@ static int batched_impl(const hmx_pair *pairs, int npairs, const SetupTaps &st, 
     HMX_REQUIRE(av.rx <= 1 && av.ry <= 1 && av.nd <= HS_MAX_TAPS,
                 "hmx_hs_batched: averaging stencil must lie within radius 1 with <= 9 taps");
     if (variant < 0) {  // autotuned choice for this size signature, else default
-        variant = 1;
+        variant = 8;  // stepped, double-buffered, reciprocal quotient
         k = 8;
         std::vector<int> sig;
         for (int i = 0; i < npairs; ++i) {
@@ -524,7 +633,7 @@ extern "C" int hmx_hs_batched_tune(const hmx_pair *pairs, int npairs, int ndx, c
     HMX_HIP_TRY(hipEventCreate(&e0));
     HMX_HIP_TRY(hipEventCreate(&e1));
     float best = 1e30f;
-    int bv = 1, bk = 8;
+    int bv = 8, bk = 8;
     for (int vi = 0; vi < kNumBVariants && !rc; ++vi) {
         for (int k = 2; k <= kBVariants[vi].kmax && !rc; k *= 2) {
             rc = batched_impl(pairs, npairs, st, av, iters, vi, k, HMX_BATCH_SYNC,

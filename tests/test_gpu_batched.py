@@ -82,10 +82,12 @@ def test_tune_then_run_matches():
 
 
 def test_forced_barrier_timeout_drains_and_recovers():
+    """The persistent form's bounded barrier wait (variant 1; the stepped
+    variants have no grid barrier)."""
     from hindemith_amd._lib import HindemithError
     pairs, B = make(12, seed=5, side=(200, 400), iters=16)
     with pytest.raises(HindemithError, match="gave up"):
-        B.run(graph=False, sync=True, spin_limit=1)
+        B.run(graph=False, sync=True, spin_limit=1, variant=1, k=8)
     out = B.run(graph=False, sync=True)
     torch.cuda.synchronize()
     check_oracle(pairs, out, 16)

@@ -22,7 +22,8 @@ struct HsImage {
     // Optional (kernels built with LOUT): for each step j (1..k), the values
     // the step linearises about on the output tile -- t and (ubar, vbar) of
     // the state before the step -- are written to (lt, lub, lvb) +
-    // (j-1)*lstride (the adjoint's recompute of a checkpoint segment).
+    // (j-1)*lstride (the adjoint's recompute of a checkpoint segment); lt may
+    // be null (t not stored).
     float *lt = nullptr, *lub = nullptr, *lvb = nullptr;
     int64_t lstride = 0;
 };
@@ -299,7 +300,7 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
                 const int gy = y0 + i / TW, gx = x0 + i % TW;
                 if (gx < x1 && gy < y1) {
                     const int64_t g = so + (int64_t)gy * a.W + gx;
-                    a.lt[g] = lin[i];
+                    if (a.lt) a.lt[g] = lin[i];  // (t is optional)
                     a.lub[g] = lin[TW * TH + i];
                     a.lvb[g] = lin[2 * TW * TH + i];
                 }

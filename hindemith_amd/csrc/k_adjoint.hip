@@ -83,15 +83,15 @@ __device__ inline float tap_apply_T(const float *__restrict__ g, int H, int W, i
 // grown by m (clipped to the image) and shrinks by one pixel per step, exactly
 // like the forward kernel's region; the accumulators gIx, gIy, gIt of the
 // tile's own pixels stay in registers and are stored once per launch; each
-// step stages only the recomputed (t, ubar, vbar) of its forward step on the
+// step stages only the recomputed (ubar, vbar) of its forward step on the
 // tile itself.
 
 struct AdjTbArgs {
     const float *Ix, *Iy, *It;
     const float *gu_in, *gv_in;  // dL/dx_hi
     float *gu_out, *gv_out;      // dL/dx_{hi-m}
-    // step j (1..m) reverses forward step hi-j, linearised about (t, ubar,
-    // vbar) at lin_hi - (j-1)*lin_step, + plane, + 2*plane
+    // step j (1..m) reverses forward step hi-j, linearised about (ubar, vbar)
+    // at lin_hi - (j-1)*lin_step and + plane (t is recomputed from them)
     const float *lin_hi;
     int64_t lin_step, plane;
     float *gIx, *gIy, *gIt;
@@ -112,7 +112,10 @@ constexpr int ADJ_TW = 64, ADJ_TH = 32, ADJ_NT = 1024, ADJ_MMAX = 8;
 // pass B gathers from neighbours, lives in LDS, double-buffered, so pass B of
 // step s and pass A of step s+1 (same region) run back to back with one
 // barrier per step.
-template <int NT, int TW, int TH, int MMAX>
+// NTAP > 0: the averaging stencil has exactly NTAP taps (straight-line
+// gather, its reads issued in groups of four so the latency of a group is
+// paid once); 0: any count up to HS_MAX_TAPS.
+template <int NT, int TW, int TH, int MMAX, int NTAP>
 __global__ __launch_bounds__(NT, 8) void hs_adj_tb_kernel(AdjTbArgs a, Taps taps) {
     constexpr int PITCH = TW + 2 * MMAX, RHM = TH + 2 * MMAX, NPX = PITCH * RHM;
     constexpr int TP = TW * TH;
@@ -184,13 +187,13 @@ __global__ __launch_bounds__(NT, 8) void hs_adj_tb_kernel(AdjTbArgs a, Taps taps
     const float *Ix = a.Ix + off, *Iy = a.Iy + off;
     float cx[P], cy[P], cd[P];
     float2 g[P];
-    float ax[PT], ay[PT], at[PT], aa[PT];
+    float ax[PT], ay[PT], at[PT], aa[PT], ct[PT];
 #pragma unroll
     for (int j = 0; j < P; ++j) {
         cx[j] = cy[j] = 0.0f;
         cd[j] = 1.0f;
         g[j] = make_float2(0.0f, 0.0f);
-        if (j < PT) ax[j] = ay[j] = at[j] = aa[j] = 0.0f;
+        if (j < PT) ax[j] = ay[j] = at[j] = aa[j] = ct[j] = 0.0f;
         if (pk[j] < 0) continue;
         const int rx = pk[j] & 0xffff, ry = (pk[j] >> 16) & 0x7ff;
         const int64_t q = (int64_t)(gy0 + ry) * W + gx0 + rx;
@@ -207,23 +210,25 @@ __global__ __launch_bounds__(NT, 8) void hs_adj_tb_kernel(AdjTbArgs a, Taps taps
             ay[j] = a.gIy[off + q];
             at[j] = a.gIt[off + q];
             aa[j] = a.ga2[off + q];
+            ct[j] = a.It[off + q];
         }
     }
 
-    // (t, ubar, vbar) of forward step hi - step at the tile slots
-    float lt[PT], lu[PT], lv[PT];
+    // (ubar, vbar) of forward step hi - step at the tile slots; its t is
+    // recomputed as ((Ix ubar + Iy vbar) + It) / den (with the reciprocal:
+    // within two ulps of the forward's quotient)
+    float lu[PT], lv[PT];
     auto load_lin = [&](int step) {
-        const float *T0 = a.lin_hi - (int64_t)(step - 1) * a.lin_step + off;
-        const float *U0 = T0 + a.plane, *V0 = T0 + 2 * a.plane;
+        const float *U0 = a.lin_hi - (int64_t)(step - 1) * a.lin_step + off;
+        const float *V0 = U0 + a.plane;
 #pragma unroll
         for (int j = 0; j < PT; ++j) {
             int pkj = pk[j];
             asm volatile("" : "+v"(pkj));  // recompute the offsets per step
-            lt[j] = lu[j] = lv[j] = 0.0f;
+            lu[j] = lv[j] = 0.0f;
             if (pkj >= 0) {
                 const int64_t q =
                     (int64_t)(gy0 + ((pkj >> 16) & 0x7ff)) * W + gx0 + (pkj & 0xffff);
-                lt[j] = T0[q];
                 lu[j] = U0[q];
                 lv[j] = V0[q];
             }
@@ -245,7 +250,8 @@ __global__ __launch_bounds__(NT, 8) void hs_adj_tb_kernel(AdjTbArgs a, Taps taps
             const float s = -fmaf(cx[j], g[j].x, cy[j] * g[j].y) * cd[j];
             GB[buf][ry * PITCH + rx] = make_float2(fmaf(cx[j], s, g[j].x), fmaf(cy[j], s, g[j].y));
             if (j < PT) {  // a tile pixel (always inside the region)
-                const float tt = lt[j], st2 = 2.0f * s * tt;
+                const float tt = (fmaf(cx[j], lu[j], cy[j] * lv[j]) + ct[j]) * cd[j];
+                const float st2 = 2.0f * s * tt;
                 ax[j] = fmaf(-st2, cx[j], fmaf(s, lu[j], fmaf(-g[j].x, tt, ax[j])));
                 ay[j] = fmaf(-st2, cy[j], fmaf(s, lv[j], fmaf(-g[j].y, tt, ay[j])));
                 at[j] += s;
@@ -260,10 +266,6 @@ __global__ __launch_bounds__(NT, 8) void hs_adj_tb_kernel(AdjTbArgs a, Taps taps
 
     for (int step = 1; step <= m; ++step) {
         const int buf = (step - 1) & 1;
-        // the next step's (t, ubar, vbar) are read before the gather, so
-        // their latency overlaps it (pass A of this step has used the
-        // current ones)
-        if (step < m) load_lin(step + 1);
         // pass B on R_{m-step}: g <- A^T g_b (gather over sources that clamp onto p)
         const int e = m - step;
         const float2 *GBb = GB[buf];
@@ -277,15 +279,31 @@ __global__ __launch_bounds__(NT, 8) void hs_adj_tb_kernel(AdjTbArgs a, Taps taps
             float au = 0.0f, av = 0.0f;
             if (!clipped || (gx > 0 && gx < W - 1 && gy > 0 && gy < H - 1)) {
                 const float2 *gb = GBb + ry * PITCH + rx;
-                // unrolled over the largest tap count, each tap tested: a
-                // straight-line 8-tap gather would hold too many reads in
-                // flight for the registers of 8 waves per SIMD
+                if (NTAP > 0) {
+                    constexpr int G = 4;
 #pragma unroll
-                for (int d = 0; d < HS_MAX_TAPS; ++d) {
-                    if (d < tp.nd) {
-                        const float2 q = gb[-tp.dl[d]];
-                        au = fmaf(tp.w[d], q.x, au);
-                        av = fmaf(tp.w[d], q.y, av);
+                    for (int d0 = 0; d0 < NTAP; d0 += G) {
+                        float2 q[G];
+#pragma unroll
+                        for (int i = 0; i < G; ++i)
+                            if (d0 + i < NTAP) q[i] = gb[-tp.dl[d0 + i]];
+#pragma unroll
+                        for (int i = 0; i < G; ++i)
+                            if (d0 + i < NTAP) {
+                                au = fmaf(tp.w[d0 + i], q[i].x, au);
+                                av = fmaf(tp.w[d0 + i], q[i].y, av);
+                            }
+                        // the next group's reads stay behind this group's use
+                        asm volatile("" ::: "memory");
+                    }
+                } else {
+#pragma unroll
+                    for (int d = 0; d < HS_MAX_TAPS; ++d) {
+                        if (d < tp.nd) {
+                            const float2 q = gb[-tp.dl[d]];
+                            au = fmaf(tp.w[d], q.x, au);
+                            av = fmaf(tp.w[d], q.y, av);
+                        }
                     }
                 }
             } else {
@@ -304,6 +322,8 @@ __global__ __launch_bounds__(NT, 8) void hs_adj_tb_kernel(AdjTbArgs a, Taps taps
             g[j] = make_float2(au, av);
         }
         if (step < m) {
+            // (read after the gather: holding them across it would spill)
+            load_lin(step + 1);
             // GB[buf ^ 1] was last read in pass B of step - 1, before the
             // previous barrier
             pass_a(step + 1, buf ^ 1);
@@ -442,9 +462,9 @@ extern "C" int hmx_hs_ckpt_count(int iters, int ckpt_every) {
 extern "C" int64_t hmx_hs_adjoint_workspace(int B, int H, int W, int ckpt_every) {
     const int64_t n = (int64_t)B * H * W;
     // Ix Iy It | gIx gIy gIt | 2 gradient ping-pong pairs | 2 recompute state
-    // ping-pong pairs | per-pixel alpha terms | (t, ubar, vbar) of every step
-    // of one segment | alpha^2 per pair | fp64 partial sums of the alpha terms
-    const int64_t floats = 3 * n + 3 * n + 4 * n + 4 * n + n + 3 * n * std::max(ckpt_every, 1);
+    // ping-pong pairs | per-pixel alpha terms | (ubar, vbar) of every step of
+    // one segment | alpha^2 per pair | fp64 partial sums of the alpha terms
+    const int64_t floats = 3 * n + 3 * n + 4 * n + 4 * n + n + 2 * n * std::max(ckpt_every, 1);
     return floats * 4 + (int64_t)B * 4 + 256 + (int64_t)B * GALPHA_CHUNKS * 8;
 }
 
@@ -521,8 +541,8 @@ extern "C" int hmx_hs_adjoint(const float *I1, const float *I2, int B, int H, in
     float *g[2][2] = {{ws + 6 * n, ws + 7 * n}, {ws + 8 * n, ws + 9 * n}};
     float *rs[2][2] = {{ws + 10 * n, ws + 11 * n}, {ws + 12 * n, ws + 13 * n}};
     float *ga2 = ws + 14 * n;  // per-pixel running sums of the alpha term
-    float *lin = ws + 15 * n;  // slot j: t, ubar, vbar of the segment's step j
-    float *alpha2 = lin + 3 * n * ckpt_every;
+    float *lin = ws + 15 * n;  // slot j: ubar, vbar of the segment's step j
+    float *alpha2 = lin + 2 * n * ckpt_every;
     double *partial = (double *)(((uintptr_t)(alpha2 + B) + 255) & ~(uintptr_t)255);
 
     hipLaunchKernelGGL(alpha_sq_kernel, dim3((B + 63) / 64), dim3(64), 0, s, alpha, alpha2, B);
@@ -555,17 +575,17 @@ extern "C" int hmx_hs_adjoint(const float *I1, const float *I2, int B, int H, in
     for (int c = nck - 1; c >= 0; --c) {
         const int len = (iters - c * ckpt_every) < ckpt_every ? iters - c * ckpt_every : ckpt_every;
         // recompute the segment's len steps from its checkpoint, k steps per
-        // launch, keeping each step's (t, ubar, vbar); the states themselves
+        // launch, keeping each step's (ubar, vbar); the states themselves
         // are bit-identical to the forward pass and only ping-pong here
         const float *cu = ckpt + (int64_t)(2 * c) * n, *cv = ckpt + (int64_t)(2 * c + 1) * n;
         int rcur = 0;
         for (int j0 = 0; j0 < len;) {
             const int kk = std::min(ADJ_RECOMPUTE_K, len - j0);
-            float *lt = lin + (int64_t)3 * j0 * n;
+            float *lub = lin + (int64_t)2 * j0 * n;
             rc = launch_hs_tb(ADJ_RECOMPUTE_VARIANT, Ix, Iy, It, nullptr, 0.0f, H, W, T.av, kk,
                               j0 == 0 ? cu : rs[rcur][0], j0 == 0 ? cv : rs[rcur][1],
-                              rs[rcur ^ 1][0], rs[rcur ^ 1][1], s, B, alpha2, lt, lt + n,
-                              lt + 2 * n, 3 * n);
+                              rs[rcur ^ 1][0], rs[rcur ^ 1][1], s, B, alpha2, nullptr, lub,
+                              lub + n, 2 * n);
             if (rc) return rc;
             rcur ^= 1;
             j0 += kk;
@@ -574,15 +594,19 @@ extern "C" int hmx_hs_adjoint(const float *I1, const float *I2, int B, int H, in
         for (int hi = len; hi > 0;) {
             const int mm = std::min(ADJ_MMAX, hi);
             a.m = mm;
-            a.lin_hi = lin + (int64_t)3 * (hi - 1) * n;
-            a.lin_step = (int64_t)3 * n;
+            a.lin_hi = lin + (int64_t)2 * (hi - 1) * n;
+            a.lin_step = (int64_t)2 * n;
             a.plane = n;
             a.gu_in = g[cur][0];
             a.gv_in = g[cur][1];
             a.gu_out = g[cur ^ 1][0];
             a.gv_out = g[cur ^ 1][1];
-            hipLaunchKernelGGL((hs_adj_tb_kernel<ADJ_NT, ADJ_TW, ADJ_TH, ADJ_MMAX>),
-                               dim3((unsigned)ntiles, B), dim3(ADJ_NT), 0, s, a, T.av);
+            if (T.av.nd == 8)
+                hipLaunchKernelGGL((hs_adj_tb_kernel<ADJ_NT, ADJ_TW, ADJ_TH, ADJ_MMAX, 8>),
+                                   dim3((unsigned)ntiles, B), dim3(ADJ_NT), 0, s, a, T.av);
+            else
+                hipLaunchKernelGGL((hs_adj_tb_kernel<ADJ_NT, ADJ_TW, ADJ_TH, ADJ_MMAX, 0>),
+                                   dim3((unsigned)ntiles, B), dim3(ADJ_NT), 0, s, a, T.av);
             HMX_LAUNCH_CHECK("hs_adj_tb");
             cur ^= 1;
             hi -= mm;

@@ -67,7 +67,7 @@ __global__ __launch_bounds__(NT, LOUT ? 8 : 1) void hs_tb_kernel(HsArgs a, Taps 
     im.W = a.W;
     im.tiles_x = a.tiles_x;
     if (LOUT) {
-        im.lt = a.lt + off;
+        im.lt = a.lt ? a.lt + off : nullptr;
         im.lub = a.lub + off;
         im.lvb = a.lvb + off;
         im.lstride = a.lstride;
@@ -181,10 +181,12 @@ static constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 static std::mutex g_tune_mu;
 static std::map<std::pair<int, int>, std::pair<int, int>> g_tuned;  // (H,W) -> (variant,k)
 
+// Untuned sizes: double-buffered window with the reciprocal quotient, k = 8
+// (the fastest variant at 512^2 .. 3840x2160 in the sweeps).
 static void default_choice(int H, int W, int &variant, int &k) {
     (void)H;
     (void)W;
-    variant = 3;
+    variant = 24;
     k = 8;
 }
 
@@ -218,7 +220,7 @@ int launch_hs_tb(int variant, const float *Ix, const float *Iy, const float *It,
     const int tiles_y = (H + V.th - 1) / V.th;
     a.ntiles = a.tiles_x * tiles_y;
     auto kern = V.kernel[tap_class(taps.nd)];
-    if (lt) {
+    if (lt || lub) {
         // linearisation output exists only for the 1024-thread 64x32 k<=8 shape
         static void (*const kLinKernels[5])(HsArgs, Taps) = {
             hs_tb_kernel<1024, 64, 32, 8, 4, true>, hs_tb_kernel<1024, 64, 32, 8, 5, true>,

@@ -254,8 +254,8 @@ int hmx_hs_solve_f64(const double *I1, const double *I2, int B, int H, int W,
                      int iters, double *u, double *v, double *workspace, void *stream);
 
 /* ---------------------------------------------------------------------- */
-/* a14. Batched persistent solve of many pairs of uneven sizes (no          */
-/* counterpart in the reference).                                           */
+/* a14. Batched solve of many pairs of uneven sizes (no counterpart in the  */
+/* reference).                                                              */
 
 /* One pair: device frames I1, I2 (H x W), initial/final flow u, v, and a
  * work area of 5*H*W floats (Ix, Iy, It and the ping-pong planes). */
@@ -273,14 +273,18 @@ struct hmx_pair {
 #define HMX_BATCH_GRAPH 1 /* capture the launch in a hipGraph, replay on later calls */
 #define HMX_BATCH_SYNC 2  /* synchronise and report a barrier time-out */
 
-/* Setup + iters Jacobi iterations of every pair in ONE persistent launch: a
- * grid no larger than the GPU holds at once walks the tiles of all pairs,
- * with a grid-wide barrier between phases of k iterations.  Each of the 8
- * XCDs owns a contiguous share of the tile list; its work-groups claim tiles
- * from that share through a per-phase counter.  variant = -1 uses the
- * autotuned (or default) tile shape and k.  Barrier waits give up after
- * spin_limit polls (<= 0: 2^24, about 3 s) and then every work-group exits;
- * with HMX_BATCH_SYNC the call returns HMX_ERR_TIMEOUT in that case.
+/* Setup + iters Jacobi iterations of every pair over one list of all pairs'
+ * tiles, in phases of k iterations.  Two forms:
+ *   stepped (variants 8-11, the default): one launch per phase over every
+ *     tile (plus a setup launch and, after an odd number of phases, a copy);
+ *   persistent (variants 0-7): ONE launch whose grid, no larger than the GPU
+ *     holds at once, walks the list with a grid-wide barrier between phases;
+ *     each of the 8 XCDs owns a contiguous share of the list and its
+ *     work-groups claim tiles from it through a per-phase counter.  Barrier
+ *     waits give up after spin_limit polls (<= 0: 2^24, about 3 s) and then
+ *     every work-group exits; with HMX_BATCH_SYNC the call returns
+ *     HMX_ERR_TIMEOUT in that case.
+ * variant = -1 uses the autotuned (or default) variant and k.
  * Launch state (descriptors, barrier words, graph) is cached per signature;
  * calls that overlap on one stream must not share pairs. */
 int hmx_hs_batched(const struct hmx_pair *pairs, int npairs,
@@ -303,7 +307,8 @@ int hmx_hs_batched_variant_info(int variant, int *tile_w, int *tile_h, int *thre
 /* Work-groups of a batched variant (averaging stencil of ntaps taps) resident per CU:
  * the runtime's occupancy figure, the bound from the kernel's registers and LDS, and
  * the CU count.  The persistent grid is min(api, own) x cus rounded down to a
- * multiple of 8 (the XCD count). */
+ * multiple of 8 (the XCD count); the stepped variants report own = api (they have no
+ * residency requirement). */
 int hmx_hs_batched_resident(int variant, int ntaps, int *api_per_cu, int *own_per_cu,
                             int *cus);
 /* Free every cached batched launch (descriptors, graphs). */

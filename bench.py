@@ -226,11 +226,16 @@ def extra_bandwidth(K, S):
     ip, ix, d = S.powerlaw_csr(N, np.random.default_rng(1))
     tip, tix, td = (torch.from_numpy(a).cuda() for a in (ip, ix, d))
     nnz = int(ip[-1])
+    rb = K.CSRRowBlocks(ip, "cuda")  # built once per matrix, on the host
+    ms = _time_graph_ms(lambda: K.csr_spmv_rowblocks(tip, tix, td, xf, rb, out=yf))
     bins = K.CSRBins(ip, "cuda")
-    ms = _time_graph_ms(lambda: K.csr_spmv_binned(tip, tix, td, xf, bins, out=yf))
+    ms_bin = _time_graph_ms(lambda: K.csr_spmv_binned(tip, tix, td, xf, bins, out=yf),
+                            launches=20)
     ms_vec = _time_graph_ms(lambda: K.csr_spmv(tip, tix, td, xf, out=yf), launches=20)
     out["csr_powerlaw_zipf1.5"] = {"nnz": nnz, "us": round(ms * 1e3, 2),
                                    "GBps": round((8 * nnz + 12 * N) / ms / 1e6, 1),
+                                   "form": "row blocks (<= 2048 entries per work-group)",
+                                   "us_row_binned": round(ms_bin * 1e3, 2),
                                    "us_unbinned": round(ms_vec * 1e3, 2)}
     return out
 

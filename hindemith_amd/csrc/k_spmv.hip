@@ -90,6 +90,57 @@ int launch_csr(int M, int64_t nnz, const int *indptr, const int *indices, const 
 static const int kBinMax[4] = {4, 16, 64, 1 << 30};
 static const int kBinLanes[4] = {4, 16, 64, 64};
 
+// ---------------------------------------------------------------------------
+// Row-block ("stream") CSR for irregular row lengths.  The rows are cut once,
+// on the host, into blocks of consecutive rows holding at most RB_CAP
+// entries and RB_NT rows (a longer row is a block of its own).  A work-group
+// takes one block: its entries are one contiguous span of data / indices,
+// read fully coalesced whatever the row lengths, and each product
+// data[k] * x[indices[k]] goes to LDS; then the block's rows split the
+// work-group's threads between them (L lanes per row, L the largest power of
+// two with rows * L <= RB_NT) and fold their products from LDS in a fixed
+// order.  Every work-group moves about the same number of entries.  A long
+// row's block strides the row over all threads (fma partials, fixed tree).
+constexpr int RB_NT = 256, RB_CAP = 2048;
+
+__global__ __launch_bounds__(RB_NT) void csr_rowblock_kernel(const int *__restrict__ blocks,
+                                                             const int *__restrict__ indptr,
+                                                             const int *__restrict__ indices,
+                                                             const float *__restrict__ data,
+                                                             const float *__restrict__ x,
+                                                             float *__restrict__ y) {
+    __shared__ float prod[RB_CAP];
+    __shared__ float red[RB_NT];
+    const int tid = threadIdx.x;
+    const int r0 = blocks[blockIdx.x], r1 = blocks[blockIdx.x + 1];
+    const int beg = indptr[r0], end = indptr[r1];
+    const int n = end - beg, nrows = r1 - r0;
+    if (n > RB_CAP) {  // one long row
+        float acc = 0.0f;
+        for (int k = beg + tid; k < end; k += RB_NT) acc = fmaf(data[k], x[indices[k]], acc);
+        red[tid] = acc;
+        __syncthreads();
+        for (int w = RB_NT / 2; w > 0; w >>= 1) {
+            if (tid < w) red[tid] += red[tid + w];
+            __syncthreads();
+        }
+        if (tid == 0) y[r0] = red[0];
+        return;
+    }
+    for (int k = tid; k < n; k += RB_NT) prod[k] = data[beg + k] * x[indices[beg + k]];
+    __syncthreads();
+    int L = 1;
+    while (L < 64 && 2 * L * nrows <= RB_NT) L *= 2;
+    const int row = tid / L, lane = tid - row * L;  // a row's lanes sit in one wave
+    float acc = 0.0f;
+    if (row < nrows) {
+        const int a = indptr[r0 + row] - beg, e = indptr[r0 + row + 1] - beg;
+        for (int k = a + lane; k < e; k += L) acc += prod[k];
+    }
+    for (int off = L / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (row < nrows && lane == 0) y[r0 + row] = acc;
+}
+
 }  // namespace hmx
 
 using namespace hmx;
@@ -140,5 +191,42 @@ extern "C" int hmx_csr_spmv_binned(int M, const int *indptr, const int *indices,
         if (rc) return rc;
         off += bin_counts[b];
     }
+    return HMX_OK;
+}
+
+extern "C" int hmx_csr_row_blocks(int M, const int *indptr_host, int *blocks_host,
+                                  int *nblocks) {
+    HMX_REQUIRE(M >= 0 && indptr_host && blocks_host && nblocks,
+                "hmx_csr_row_blocks: bad args");
+    int nb = 0, r = 0;
+    blocks_host[0] = 0;
+    while (r < M) {
+        const int start = r;
+        if (indptr_host[r + 1] - indptr_host[r] > RB_CAP) {
+            ++r;  // a long row alone
+        } else {
+            int64_t n = 0;
+            while (r < M && r - start < RB_NT &&
+                   n + (indptr_host[r + 1] - indptr_host[r]) <= RB_CAP) {
+                n += indptr_host[r + 1] - indptr_host[r];
+                ++r;
+            }
+        }
+        blocks_host[++nb] = r;
+    }
+    *nblocks = nb;
+    return HMX_OK;
+}
+
+extern "C" int hmx_csr_spmv_rowblocks(int M, const int *indptr, const int *indices,
+                                      const float *data, const float *x, float *y,
+                                      const int *blocks, int nblocks, void *stream) {
+    HMX_REQUIRE(M >= 0 && nblocks >= 0 && indptr && x && y && (blocks || nblocks == 0),
+                "hmx_csr_spmv_rowblocks: null argument");
+    HMX_REQUIRE((const void *)x != (const void *)y, "hmx_csr_spmv_rowblocks: x and y must differ");
+    if (nblocks == 0) return HMX_OK;
+    hipLaunchKernelGGL(csr_rowblock_kernel, dim3(nblocks), dim3(RB_NT), 0, as_stream(stream),
+                       blocks, indptr, indices, data, x, y);
+    HMX_LAUNCH_CHECK("csr_rowblock");
     return HMX_OK;
 }

@@ -122,6 +122,32 @@ class CSRBins:
         self.bins = torch.from_numpy(bins[:max(M, 1)]).to(device)
 
 
+class CSRRowBlocks:
+    """Row blocks of a CSR matrix (<= 2048 entries, <= 256 rows each; built
+    once per matrix on the host) for hmx_csr_spmv_rowblocks."""
+
+    def __init__(self, indptr_host, device):
+        ip = np.ascontiguousarray(indptr_host, np.int32)
+        M = len(ip) - 1
+        blocks = np.empty(M + 1, np.int32)
+        nb = ctypes.c_int(0)
+        check(lib().hmx_csr_row_blocks(M, ip.ctypes.data_as(ctypes.c_void_p),
+                                       blocks.ctypes.data_as(ctypes.c_void_p),
+                                       ctypes.byref(nb)), "csr_row_blocks")
+        self.M = M
+        self.nblocks = nb.value
+        self.blocks = torch.from_numpy(blocks[:nb.value + 1].copy()).to(device)
+
+
+def csr_spmv_rowblocks(indptr, indices, data, x, rb, out=None):
+    """y = A * x with the row-block launch (rb: CSRRowBlocks of A)."""
+    y = torch.empty(rb.M, device=x.device, dtype=torch.float32) if out is None else out
+    check(lib().hmx_csr_spmv_rowblocks(rb.M, _ptr(indptr), _ptr(indices), _ptr(data), _ptr(x),
+                                       _ptr(y), _ptr(rb.blocks), rb.nblocks, stream_ptr()),
+          "csr_spmv_rowblocks")
+    return y
+
+
 def csr_spmv_binned(indptr, indices, data, x, bins, out=None):
     M = indptr.numel() - 1
     y = torch.empty(M, device=x.device, dtype=torch.float32) if out is None else out

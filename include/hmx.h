@@ -128,6 +128,18 @@ int hmx_csr_spmv_binned(int M, const int *indptr, const int *indices, const floa
                         const float *x, float *y, const int *bins, const int *bin_counts,
                         void *stream);
 
+/* Row-block CSR SpMV for irregular row lengths (same product as SpmvCSR).
+ * hmx_csr_row_blocks cuts the rows, on the host, into blocks of consecutive
+ * rows with at most 2048 entries and 256 rows (a longer row alone):
+ * blocks_host (capacity M + 1) receives the nblocks + 1 block start rows.
+ * hmx_csr_spmv_rowblocks runs one work-group per block (blocks: the same
+ * array on the device); entries are streamed coalesced and each row's
+ * products are added in a fixed order (error within the multi-lane bound). */
+int hmx_csr_row_blocks(int M, const int *indptr_host, int *blocks_host, int *nblocks);
+int hmx_csr_spmv_rowblocks(int M, const int *indptr, const int *indices, const float *data,
+                           const float *x, float *y, const int *blocks, int nblocks,
+                           void *stream);
+
 /* a7. out[0] = sum of x[0..n) (sum2d, Array2DSum.cpp:12-250).  Deterministic
  *     two-pass reduction with fp64 accumulation.  partials: device scratch of
  *     HMX_SUM_PARTIALS doubles.  out: device float. */

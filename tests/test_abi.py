@@ -49,3 +49,28 @@ def test_missing_library_fails_loudly(monkeypatch):
     monkeypatch.setattr(_lib, "LIB_PATH", "/nonexistent/libhmx.so")
     with pytest.raises(_lib.HindemithError, match="no CPU fallback"):
         _lib.lib()
+
+
+def test_csr_row_blocks_host():
+    """The host-side row blocking behind hmx_csr_spmv_rowblocks: blocks cover
+    the rows in order, hold at most 2048 entries and 256 rows, and a longer
+    row stands alone (no GPU involved)."""
+    import numpy as np
+    from hindemith_amd._lib import lib
+    rng = np.random.default_rng(5)
+    for lens in (np.array([], np.int64), np.array([3]), np.array([5000]),
+                 rng.integers(0, 40, 10000), np.concatenate([rng.integers(0, 5, 600), [4000],
+                                                             np.zeros(700, np.int64), [2049]])):
+        M = len(lens)
+        ip = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+        blocks = np.empty(M + 1, np.int32)
+        nb = ctypes.c_int(-1)
+        assert lib().hmx_csr_row_blocks(M, ip.ctypes.data_as(ctypes.c_void_p),
+                                        blocks.ctypes.data_as(ctypes.c_void_p),
+                                        ctypes.byref(nb)) == 0
+        b = blocks[:nb.value + 1]
+        assert b[0] == 0 and b[-1] == M and np.all(np.diff(b) >= 1)
+        for r0, r1 in zip(b[:-1], b[1:]):
+            n = ip[r1] - ip[r0]
+            assert r1 - r0 <= 256
+            assert n <= 2048 or r1 - r0 == 1

@@ -114,6 +114,44 @@ def test_csr_binned_matches_vector(K):
     assert np.all(np.abs(y - ref) <= (lens + 2) * np.finfo(np.float32).eps * absdot + 1e-30)
 
 
+@pytest.mark.parametrize("case", ["random", "powerlaw", "long_rows", "tiny", "empty_rows"])
+def test_csr_rowblocks(K, case):
+    """Row-block CSR SpMV: within the multi-lane bound of the fp64 product, rows
+    longer than a block included, and bit-identical on a second call."""
+    rng = np.random.default_rng(12)
+    if case == "random":
+        M, N = 3001, 2500
+        indptr, indices, data = _random_csr(M, N, rng)
+    elif case == "powerlaw":
+        M = N = 20000
+        indptr, indices, data = S.powerlaw_csr(N, rng)
+    elif case == "long_rows":  # rows of 5000 and 3000 entries among short ones
+        M, N = 40, 6000
+        indptr, indices, data = _random_csr(M, N, rng, max_len=10)
+        lens = np.diff(indptr)
+        lens[3], lens[17] = 5000, 3000
+        indptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+        indices = rng.integers(0, N, int(indptr[-1])).astype(np.int32)
+        data = rng.normal(size=int(indptr[-1])).astype(np.float32)
+    elif case == "tiny":
+        M, N = 1, 1
+        indptr, indices, data = (np.array([0, 1], np.int32), np.array([0], np.int32),
+                                 np.array([2.5], np.float32))
+    else:
+        M, N = 700, 900
+        indptr, indices, data = _random_csr(M, N, rng, empty_frac=0.8)
+    x = rng.uniform(-1, 1, N).astype(np.float32)
+    rb = K.CSRRowBlocks(indptr, "cuda")
+    ip, ix, dd, dx = dev(indptr), dev(indices), dev(data), dev(x)
+    y1 = host(K.csr_spmv_rowblocks(ip, ix, dd, dx, rb))
+    y2 = host(K.csr_spmv_rowblocks(ip, ix, dd, dx, rb))
+    assert y1.tobytes() == y2.tobytes()
+    ref = oracle.csr_spmv(indptr, indices, data, x, dtype=np.float64)
+    absdot = oracle.csr_spmv(indptr, indices, np.abs(data), np.abs(x), dtype=np.float64)
+    lens = np.diff(indptr)
+    assert np.all(np.abs(y1 - ref) <= (lens + 2) * np.finfo(np.float32).eps * absdot + 1e-30)
+
+
 @pytest.mark.parametrize("n", [0, 1, 1000, 2048 * 2048 + 17])
 def test_sum2d(K, n):
     rng = np.random.default_rng(n)

@@ -93,7 +93,7 @@ __global__ __launch_bounds__(NT) void hs_powers_kernel(PowArgs a) {
     const int x1 = min(x0 + TW, W), y1 = min(y0 + TH, H);
     const int gx0 = max(x0 - m, 0), gy0 = max(y0 - m, 0);
     const int gx1 = min(x1 + m, W), gy1 = min(y1 + m, H);
-    const int RW = gx1 - gx0, npx = RW * (gy1 - gy0);
+    const int RW = gx1 - gx0, RH = gy1 - gy0;
     // a scalar of another dtype is rounded to float32 first
     float lam = a.lam_val;
     if (a.lam_dev) {
@@ -106,22 +106,31 @@ __global__ __launch_bounds__(NT) void hs_powers_kernel(PowArgs a) {
     }
     const float nlam = -lam;
 
+    // Thread tid owns window cells c = tid + j*NT (the pitch-PITCH grid), so a
+    // neighbour of cell c is c +- 1 or c +- PITCH: constant LDS offsets.  pk
+    // packs L << 27 | ry << 16 | rx for a cell in the region, L = m - (its
+    // distance outside the tile) = the last power that updates it (the
+    // region of power j is the tile grown by m - j, clipped to the image);
+    // -1 for cells outside the region.
+    const bool clipped = (x0 - m < 0) || (y0 - m < 0) || (x1 + m > W) || (y1 + m > H);
     int pk[P];
     float cx[P], cy[P];
 #pragma unroll
     for (int j = 0; j < P; ++j) {
-        const int i = threadIdx.x + j * NT;
+        const int c = threadIdx.x + j * NT;
+        const int ry = c / PITCH, rx = c - ry * PITCH;
         pk[j] = -1;
         cx[j] = cy[j] = 0.0f;
-        if (i < npx) {
-            const int ry = i / RW, rx = i - ry * RW;
-            pk[j] = (ry << 16) | rx;
-            const int64_t g = (int64_t)(gy0 + ry) * W + gx0 + rx;
+        if (c < NPX && rx < RW && ry < RH) {
+            const int gx = gx0 + rx, gy = gy0 + ry;
+            const int dx = max(max(x0 - gx, gx - x1 + 1), 0);
+            const int dy = max(max(y0 - gy, gy - y1 + 1), 0);
+            pk[j] = ((m - max(dx, dy)) << 27) | (ry << 16) | rx;
+            const int64_t g = (int64_t)gy * W + gx;
             cx[j] = a.Ix[g];
             cy[j] = a.Iy[g];
             const float u = a.u0[g], v = a.v0[g];
-            win[0][ry * PITCH + rx] = make_float2(u, v);
-            const int gx = gx0 + rx, gy = gy0 + ry;
+            win[0][c] = make_float2(u, v);
             if (a.copy_r && gx >= x0 && gx < x1 && gy >= y0 && gy < y1) {
                 a.V0[g] = u;
                 a.V1[g] = v;
@@ -133,55 +142,78 @@ __global__ __launch_bounds__(NT) void hs_powers_kernel(PowArgs a) {
     for (int j = 1; j <= m; ++j) {
         const float2 *rd = win[(j - 1) & 1];
         float2 *wr = win[j & 1];
-        const int e = m - j;
-        const int rx0 = max(x0 - e, 0) - gx0, ry0 = max(y0 - e, 0) - gy0;
-        const int rx1 = min(x1 + e, W) - gx0, ry1 = min(y1 + e, H) - gy0;
+        const int act_min = j << 27, tile_min = m << 27;
         float *V0s = a.V0 + (int64_t)(a.s0 + j - 1) * a.plane;
         float *V1s = a.V1 + (int64_t)(a.s0 + j - 1) * a.plane;
 #pragma unroll
         for (int q = 0; q < P; ++q) {
-            int pkq = pk[q];
-            asm volatile("" : "+v"(pkq));  // keep per-pixel addresses out of registers
-            const int rx = pkq & 0xffff, ry = pkq >> 16;
-            if (pkq < 0 || rx < rx0 || rx >= rx1 || ry < ry0 || ry >= ry1) continue;
-            const int gx = gx0 + rx, gy = gy0 + ry;
-            const float2 *c = rd + ry * PITCH + rx;
-            const float2 mid = c[0];
+            const int pkq = pk[q];
+            if (pkq < act_min) continue;
+            const int c = threadIdx.x + q * NT;
+            const float2 mid = rd[c];
             const float ix = cx[q], iy = cy[q];
             const float p1 = ix * mid.x, p2 = iy * mid.y;
             const float dot = p1 + p2;
             float ur = ix * dot, vr = iy * dot;
-            if (gx > 0) {
-                const float2 nb = c[-1];
-                ur = fmaf(nlam, nb.x, ur);
-                vr = fmaf(nlam, nb.y, vr);
+            if (!clipped) {
+                // an updated cell is at least one cell inside the region on
+                // every side and the region inside the image: all four
+                // neighbours exist; reads issued together
+                const float2 nl = rd[c - 1], nr = rd[c + 1], nu = rd[c - PITCH],
+                             nd = rd[c + PITCH];
+                ur = fmaf(nlam, nl.x, ur);
+                vr = fmaf(nlam, nl.y, vr);
                 ur = fmaf(lam, mid.x, ur);
                 vr = fmaf(lam, mid.y, vr);
-            }
-            if (gx < W - 1) {
-                const float2 nb = c[1];
-                ur = fmaf(nlam, nb.x, ur);
-                vr = fmaf(nlam, nb.y, vr);
+                ur = fmaf(nlam, nr.x, ur);
+                vr = fmaf(nlam, nr.y, vr);
                 ur = fmaf(lam, mid.x, ur);
                 vr = fmaf(lam, mid.y, vr);
-            }
-            if (gy > 0) {
-                const float2 nb = c[-PITCH];
-                ur = fmaf(nlam, nb.x, ur);
-                vr = fmaf(nlam, nb.y, vr);
+                ur = fmaf(nlam, nu.x, ur);
+                vr = fmaf(nlam, nu.y, vr);
                 ur = fmaf(lam, mid.x, ur);
                 vr = fmaf(lam, mid.y, vr);
-            }
-            if (gy < H - 1) {
-                const float2 nb = c[PITCH];
-                ur = fmaf(nlam, nb.x, ur);
-                vr = fmaf(nlam, nb.y, vr);
+                ur = fmaf(nlam, nd.x, ur);
+                vr = fmaf(nlam, nd.y, vr);
                 ur = fmaf(lam, mid.x, ur);
                 vr = fmaf(lam, mid.y, vr);
+            } else {
+                int pkc = pkq;
+                asm volatile("" : "+v"(pkc));  // keep per-cell values out of registers
+                const int gx = gx0 + (pkc & 0xffff), gy = gy0 + ((pkc >> 16) & 0x7ff);
+                if (gx > 0) {
+                    const float2 nb = rd[c - 1];
+                    ur = fmaf(nlam, nb.x, ur);
+                    vr = fmaf(nlam, nb.y, vr);
+                    ur = fmaf(lam, mid.x, ur);
+                    vr = fmaf(lam, mid.y, vr);
+                }
+                if (gx < W - 1) {
+                    const float2 nb = rd[c + 1];
+                    ur = fmaf(nlam, nb.x, ur);
+                    vr = fmaf(nlam, nb.y, vr);
+                    ur = fmaf(lam, mid.x, ur);
+                    vr = fmaf(lam, mid.y, vr);
+                }
+                if (gy > 0) {
+                    const float2 nb = rd[c - PITCH];
+                    ur = fmaf(nlam, nb.x, ur);
+                    vr = fmaf(nlam, nb.y, vr);
+                    ur = fmaf(lam, mid.x, ur);
+                    vr = fmaf(lam, mid.y, vr);
+                }
+                if (gy < H - 1) {
+                    const float2 nb = rd[c + PITCH];
+                    ur = fmaf(nlam, nb.x, ur);
+                    vr = fmaf(nlam, nb.y, vr);
+                    ur = fmaf(lam, mid.x, ur);
+                    vr = fmaf(lam, mid.y, vr);
+                }
             }
-            wr[ry * PITCH + rx] = make_float2(ur, vr);
-            if (gx >= x0 && gx < x1 && gy >= y0 && gy < y1) {
-                const int64_t g = (int64_t)gy * W + gx;
+            wr[c] = make_float2(ur, vr);
+            if (pkq >= tile_min) {  // a tile cell (L = m): this power's output
+                const int64_t g =
+                    (int64_t)(gy0 + ((pkq >> 16) & 0x7ff)) * W + gx0 + (pkq & 0xffff);
                 V0s[g] = ur;
                 V1s[g] = vr;
             }

@@ -105,9 +105,25 @@ __device__ inline float setup_tap(const float *__restrict__ x, int H, int W, int
     return acc;
 }
 
+// 4-tap form: the reads issued before the fma chain.
+__device__ __forceinline__ float setup_tap4(const float *__restrict__ x, int H, int W, int r,
+                                            int c, const Taps &t) {
+    float v[4];
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+        const int xx = clampi(c + t.ox[d], 0, W - 1), yy = clampi(r + t.oy[d], 0, H - 1);
+        v[d] = x[(int64_t)yy * W + xx];
+    }
+    float acc = 0.0f;
+#pragma unroll
+    for (int d = 0; d < 4; ++d) acc = fmaf(t.w[d], v[d], acc);
+    return acc;
+}
+
 // Derivative setup of tile w of the list (Ix = Dx*I1 + Dx*I2, ...), one
-// rounding per op.
-template <int NT, int TW, int TH>
+// rounding per op.  TAP4: all three stencils have 4 taps (the canonical
+// cube), taken by the unrolled form.
+template <int NT, int TW, int TH, bool TAP4 = false>
 __device__ __forceinline__ void setup_tile(const BatchArgs &b, const SetupTaps &st, int w) {
     const PairDesc &P = b.pairs[b.tile_pair[w]];
     const int t = w - P.tile_base;
@@ -116,6 +132,18 @@ __device__ __forceinline__ void setup_tile(const BatchArgs &b, const SetupTaps &
         const int r = tyi * TH + i / TW, c = txi * TW + i % TW;
         if (r >= P.H || c >= P.W) continue;
         const int64_t p = (int64_t)r * P.W + c;
+        if (TAP4) {
+            const float a1 = setup_tap4(P.I1, P.H, P.W, r, c, st.dx);
+            const float a2 = setup_tap4(P.I2, P.H, P.W, r, c, st.dx);
+            const float b1 = setup_tap4(P.I1, P.H, P.W, r, c, st.dy);
+            const float b2 = setup_tap4(P.I2, P.H, P.W, r, c, st.dy);
+            const float t2 = setup_tap4(P.I2, P.H, P.W, r, c, st.dt);
+            const float t1 = setup_tap4(P.I1, P.H, P.W, r, c, st.dt);
+            P.Ix[p] = a1 + a2;
+            P.Iy[p] = b1 + b2;
+            P.It[p] = t2 - t1;
+            continue;
+        }
         const float a1 = setup_tap(P.I1, P.H, P.W, r, c, st.dx);
         const float a2 = setup_tap(P.I2, P.H, P.W, r, c, st.dx);
         P.Ix[p] = a1 + a2;
@@ -241,7 +269,11 @@ __global__ __launch_bounds__(NT, WPE) void hs_batched_kernel(BatchArgs b, SetupT
 
 template <int NT, int TW, int TH>
 __global__ __launch_bounds__(NT) void hs_multi_setup_kernel(BatchArgs b, SetupTaps st) {
-    setup_tile<NT, TW, TH>(b, st, xcd_remap(blockIdx.x, b.total_tiles));
+    const int w = xcd_remap(blockIdx.x, b.total_tiles);
+    if (st.dx.nd == 4 && st.dy.nd == 4 && st.dt.nd == 4)
+        setup_tile<NT, TW, TH, true>(b, st, w);
+    else
+        setup_tile<NT, TW, TH>(b, st, w);
 }
 
 template <int NT, int TW, int TH, int KMAX, int NTAP, bool DB, bool FDIV>

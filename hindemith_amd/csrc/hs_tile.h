@@ -82,6 +82,29 @@ __device__ __forceinline__ float hs_update(float ix, float iy, float it, float d
     return tt;
 }
 
+// Copies the new value q of window cell c into the ghost cells that clamp to
+// it: the neighbours across each image border the pixel lies on (pk's border
+// flags, hs_tile), corners included.  Each ghost clamps to exactly one image
+// pixel, so no two cells store to the same ghost.
+template <int PITCH>
+__device__ __forceinline__ void hs_ghost_store(float2 *w, int c, int pk, float2 q) {
+    const int e = (pk >> 12) & 15;
+    if (e == 0) return;
+    const bool l = e & 1, r = e & 2;
+    if (l) w[c - 1] = q;
+    if (r) w[c + 1] = q;
+    if (e & 4) {
+        w[c - PITCH] = q;
+        if (l) w[c - PITCH - 1] = q;
+        if (r) w[c - PITCH + 1] = q;
+    }
+    if (e & 8) {
+        w[c + PITCH] = q;
+        if (l) w[c + PITCH - 1] = q;
+        if (r) w[c + PITCH + 1] = q;
+    }
+}
+
 // LDS of one tile window (two with DB): PAD cells before and after it, so a
 // thread may read the taps of any cell it owns, updated or not, without
 // leaving the allocation.  hs_tile takes uv = the allocation + PAD.
@@ -133,19 +156,29 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
     const int tyi = t / a.tiles_x, txi = t - tyi * a.tiles_x;
     const int x0 = txi * TW, y0 = tyi * TH;
     const int x1 = min(x0 + TW, a.W), y1 = min(y0 + TH, a.H);
-    const int gx0 = max(x0 - k, 0), gy0 = max(y0 - k, 0);
-    const int gx1 = min(x1 + k, a.W), gy1 = min(y1 + k, a.H);
-    const int RW = gx1 - gx0, RH = gy1 - gy0;
-    const bool clipped = (x0 - k < 0) || (y0 - k < 0) || (x1 + k > a.W) || (y1 + k > a.H);
+    // The window's corner is image pixel (x0 - k, y0 - k) for every tile, and
+    // the region is the output tile grown by k on each side.  Where the
+    // region reaches past the image border (a clipped tile), the region
+    // cells outside the image are ghosts: each holds the value of the image
+    // pixel its coordinates clamp to, loaded here and refreshed after every
+    // step by that pixel (hs_ghost_store).  A tap that leaves the image then
+    // reads the value the reference's clamped index reads, at the window
+    // offset of any other tap, so border tiles run the same code as interior
+    // ones.  (A radius-1 stencil reads only the ghost ring next to the
+    // image; the rest of the ghosts are loaded and never read.)
+    const int gx0 = x0 - k, gy0 = y0 - k;
+    const int RW = x1 - x0 + 2 * k, RH = y1 - y0 + 2 * k;
+    const bool clipped = (gx0 < 0) || (gy0 < 0) || (x1 + k > a.W) || (y1 + k > a.H);
 
     // Thread tid owns the window cells c = tid + j*NT (j < P), so the LDS
     // address of tap d of any owned pixel is (tid + dl[d]) + j*NT: one
     // per-thread base per tap, the slot a compile-time offset.  pk packs, for
-    // a cell inside the region, L << 27 | ry << 16 | rx, where L = k - (the
-    // cell's distance outside the output tile) is the last step that updates
-    // it: the region of step it is the tile grown by k - it, clipped to the
-    // image, so the cell is updated exactly when it <= L.  Cells outside the
-    // region hold -1.
+    // a region cell inside the image, L << 27 | ry << 16 | E << 12 | rx, where
+    // L = k - (the cell's distance outside the output tile) is the last step
+    // that updates it (the region of step it is the tile grown by k - it, so
+    // the cell is updated exactly when it <= L), and E flags the image
+    // borders the pixel lies on (bit 0 left, 1 right, 2 top, 3 bottom).
+    // Ghosts and cells outside the region hold -1.
     float cx[P], cy[P], ct[P], cd[P], cr[FDIV ? P : 1];
     int pk[P];
 #pragma unroll
@@ -158,24 +191,32 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
         cd[j] = 1.0f;
         if (c < NPX && rx < RW && ry < RH) {
             const int gx = gx0 + rx, gy = gy0 + ry;
-            const int dx = max(max(x0 - gx, gx - x1 + 1), 0);
-            const int dy = max(max(y0 - gy, gy - y1 + 1), 0);
-            pk[j] = ((k - max(dx, dy)) << 27) | (ry << 16) | rx;
-            const int64_t g = (int64_t)gy * a.W + gx;
-            const float ix = a.Ix[g], iy = a.Iy[g];
-            cx[j] = ix;
-            cy[j] = iy;
-            ct[j] = a.It[g];
-            if (a.den) {
-                cd[j] = a.den[g];
+            if (gx >= 0 && gx < a.W && gy >= 0 && gy < a.H) {
+                const int dx = max(max(x0 - gx, gx - x1 + 1), 0);
+                const int dy = max(max(y0 - gy, gy - y1 + 1), 0);
+                const int e = (gx == 0) | (gx == a.W - 1) << 1 | (gy == 0) << 2 |
+                              (gy == a.H - 1) << 3;
+                pk[j] = ((k - max(dx, dy)) << 27) | (ry << 16) | (e << 12) | rx;
+                const int64_t g = (int64_t)gy * a.W + gx;
+                const float ix = a.Ix[g], iy = a.Iy[g];
+                cx[j] = ix;
+                cy[j] = iy;
+                ct[j] = a.It[g];
+                if (a.den) {
+                    cd[j] = a.den[g];
+                } else {
+                    const float m1 = ix * ix;
+                    const float s1 = a.alpha2 + m1;
+                    const float m2 = iy * iy;
+                    cd[j] = s1 + m2;
+                }
+                if (FDIV) cr[j] = hs_recip(cd[j]);
+                uv[c] = make_float2(a.u_in[g], a.v_in[g]);
             } else {
-                const float m1 = ix * ix;
-                const float s1 = a.alpha2 + m1;
-                const float m2 = iy * iy;
-                cd[j] = s1 + m2;
+                const int64_t g =
+                    (int64_t)clampi(gy, 0, a.H - 1) * a.W + clampi(gx, 0, a.W - 1);
+                uv[c] = make_float2(a.u_in[g], a.v_in[g]);
             }
-            if (FDIV) cr[j] = hs_recip(cd[j]);
-            uv[c] = make_float2(a.u_in[g], a.v_in[g]);
         }
     }
     __syncthreads();
@@ -191,7 +232,7 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
         float2 *uvw = DB ? uv + (par ^ 1) * NPX : uv;
         float nu[P], nv[P];
         bool act[P];
-        if (PREFETCH && !clipped) {
+        if (PREFETCH) {
             // Software-pipelined form: the first NPF taps of cell j+1 are read
             // (updated or not: the window is padded) before cell j is
             // computed, so LDS latency overlaps arithmetic instead of stalling
@@ -221,7 +262,11 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
                     }
                     (void)hs_update<FDIV>(cx[j], cy[j], ct[j], cd[j], FDIV ? cr[j] : 0.0f, ub,
                                           vb, nu[j], nv[j]);
-                    if (DB) uvw[tid + j * NT] = make_float2(nu[j], nv[j]);
+                    if (DB) {
+                        const float2 q = make_float2(nu[j], nv[j]);
+                        uvw[tid + j * NT] = q;
+                        if (clipped) hs_ghost_store<PITCH>(uvw, tid + j * NT, pk[j], q);
+                    }
                 }
 #pragma unroll
                 for (int d = 0; d < NPF; ++d) qa[d] = qn[d];
@@ -233,44 +278,28 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
             act[j] = pkj >= act_min;
             if (act[j]) {
                 float ub = 0.0f, vb = 0.0f;
-                if (!clipped) {
-                    // an updated cell is at least one cell inside the window
-                    // on every side, so every tap stays inside it
-                    int c = tid + j * NT;
-                    if (!SHARED) asm volatile("" : "+v"(c));
-                    const float2 *base = uvr + c;
+                // an updated cell is at least one cell inside the window on
+                // every side, so every tap stays inside it
+                int c = tid + j * NT;
+                if (!SHARED) asm volatile("" : "+v"(c));
+                const float2 *base = uvr + c;
 #pragma unroll
-                    for (int d = 0; d < (NTAP ? NTAP : HS_MAX_TAPS); ++d) {
-                        if (NTAP || d < tp.nd) {
-                            const float2 q = base[tp.dl[d]];
-                            ub = fmaf(tp.w[d], q.x, ub);
-                            vb = fmaf(tp.w[d], q.y, vb);
-                        }
-                    }
-                } else {
-                    // Opaque copy: stops the compiler from hoisting every
-                    // pixel's clamped tap addresses out of the step loop (P x
-                    // taps live registers).
-                    int pkc = pkj;
-                    asm volatile("" : "+v"(pkc));
-                    const int rx = pkc & 0xffff, ry = (pkc >> 16) & 0x7ff;
-                    const int gx = gx0 + rx, gy = gy0 + ry;
-#pragma unroll
-                    for (int d = 0; d < (NTAP ? NTAP : HS_MAX_TAPS); ++d) {
-                        if (NTAP || d < tp.nd) {
-                            const int nx = clampi(gx + tp.ox[d], 0, a.W - 1) - gx0;
-                            const int ny = clampi(gy + tp.oy[d], 0, a.H - 1) - gy0;
-                            const float2 q = uvr[ny * PITCH + nx];
-                            ub = fmaf(tp.w[d], q.x, ub);
-                            vb = fmaf(tp.w[d], q.y, vb);
-                        }
+                for (int d = 0; d < (NTAP ? NTAP : HS_MAX_TAPS); ++d) {
+                    if (NTAP || d < tp.nd) {
+                        const float2 q = base[tp.dl[d]];
+                        ub = fmaf(tp.w[d], q.x, ub);
+                        vb = fmaf(tp.w[d], q.y, vb);
                     }
                 }
                 const float tt = hs_update<FDIV>(cx[j], cy[j], ct[j], cd[j],
                                                  FDIV ? cr[j] : 0.0f, ub, vb, nu[j], nv[j]);
-                if (DB) uvw[tid + j * NT] = make_float2(nu[j], nv[j]);
+                if (DB) {
+                    const float2 q = make_float2(nu[j], nv[j]);
+                    uvw[tid + j * NT] = q;
+                    if (clipped) hs_ghost_store<PITCH>(uvw, tid + j * NT, pkj, q);
+                }
                 if (LOUT) {
-                    const int rx = pkj & 0xffff, ry = (pkj >> 16) & 0x7ff;
+                    const int rx = pkj & 0xfff, ry = (pkj >> 16) & 0x7ff;
                     const int lx = gx0 + rx - x0, ly = gy0 + ry - y0;
                     if (lx >= 0 && lx < TW && ly >= 0 && ly < TH) {
                         const int li = ly * TW + lx;
@@ -290,8 +319,13 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
         }
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < P; ++j)
-            if (act[j]) uv[tid + j * NT] = make_float2(nu[j], nv[j]);
+        for (int j = 0; j < P; ++j) {
+            if (act[j]) {
+                const float2 q = make_float2(nu[j], nv[j]);
+                uv[tid + j * NT] = q;
+                if (clipped) hs_ghost_store<PITCH>(uv, tid + j * NT, pk[j], q);
+            }
+        }
         if (LOUT) {
             // lin is rewritten only in the next step's compute, after the
             // barrier below
@@ -320,7 +354,7 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
 #pragma unroll
     for (int j = 0; j < P; ++j) {
         if (pk[j] < 0) continue;
-        const int rx = pk[j] & 0xffff, ry = (pk[j] >> 16) & 0x7ff;
+        const int rx = pk[j] & 0xfff, ry = (pk[j] >> 16) & 0x7ff;
         const int gx = gx0 + rx, gy = gy0 + ry;
         if (gx >= x0 && gx < x1 && gy >= y0 && gy < y1) {
             const int64_t g = (int64_t)gy * a.W + gx;

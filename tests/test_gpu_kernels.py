@@ -271,6 +271,28 @@ def test_hs_jacobi_tap_classes_bitexact(K, kind):
         assert np.array_equal(host(u), ref_u) and np.array_equal(host(v), ref_v), (kind, vi, k)
 
 
+@pytest.mark.parametrize("shape", [(1, 1), (1, 70), (70, 1), (2, 3), (33, 65), (40, 129)])
+def test_hs_jacobi_border_clamp_bitexact(K, shape):
+    """Border tiles read clamped neighbours from ghost cells: a stencil with nine
+    distinct weights in scrambled order tells every clamped tap apart (a ghost
+    that held the wrong pixel, or a missing corner copy, changes the sum)."""
+    I1, I2, _, _ = S.hs_pair(*shape, seed=11)
+    Ix, Iy, It = oracle.hs_setup(I1, I2, TAPS["Dx"], TAPS["Dy"], TAPS["Dt"])
+    order = [4, 0, 8, 2, 6, 1, 7, 3, 5]
+    w = np.array([0.02, 0.05, 0.07, 0.11, 0.13, 0.17, 0.19, 0.1, 0.16], np.float32)[order]
+    ox = np.array([-1, 0, 1] * 3, np.int32)[order]
+    oy = np.repeat(np.array([-1, 0, 1], np.int32), 3)[order]
+    avg = (w, ox, oy)
+    rng = np.random.default_rng(5)
+    u0 = rng.normal(size=shape).astype(np.float32)
+    v0 = rng.normal(size=shape).astype(np.float32)
+    ref_u, ref_v = oracle.hs_jacobi(Ix, Iy, It, 0.6, avg, 10, u0, v0)
+    for vi, k in _variants(K):
+        u, v = dev(u0), dev(v0)
+        K.hs_jacobi(dev(Ix), dev(Iy), dev(It), 0.6, avg, 10, u, v, variant=vi, k=k)
+        assert np.array_equal(host(u), ref_u) and np.array_equal(host(v), ref_v), (vi, k)
+
+
 def _bitwise_equal_nan(got, ref):
     """Bitwise equal (zero signs included); NaN where the reference is NaN."""
     nan = np.isnan(ref)

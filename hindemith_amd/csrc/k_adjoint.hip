@@ -117,11 +117,14 @@ constexpr int ADJ_TW = 64, ADJ_TH = 32, ADJ_NT = 1024, ADJ_MMAX = 8;
 // paid once); 0: any count up to HS_MAX_TAPS.
 template <int NT, int TW, int TH, int MMAX, int NTAP>
 __global__ __launch_bounds__(NT, 8) void hs_adj_tb_kernel(AdjTbArgs a, Taps taps) {
-    constexpr int PITCH = TW + 2 * MMAX, RHM = TH + 2 * MMAX, NPX = PITCH * RHM;
+    // LDS window: the tile grown by MMAX, plus one column and one row, so the
+    // ghost gathers of pass B (up to two pixels past the image) stay inside
+    // it on every side
+    constexpr int PITCH = TW + 2 * MMAX + 1, RHM = TH + 2 * MMAX + 1, NPX = PITCH * RHM;
     constexpr int TP = TW * TH;
     static_assert(TP % NT == 0, "tile pixels must split evenly over the threads");
     constexpr int PT = TP / NT;
-    constexpr int PH = (NPX - TP + NT - 1) / NT;
+    constexpr int PH = ((TW + 2 * MMAX) * (TH + 2 * MMAX) - TP + NT - 1) / NT;
     constexpr int P = PT + PH;
     __shared__ float2 GB[2][NPX];  // (g_ubar, g_vbar) on the current region
 
@@ -139,17 +142,29 @@ __global__ __launch_bounds__(NT, 8) void hs_adj_tb_kernel(AdjTbArgs a, Taps taps
     const bool clipped = (x0 - m < 0) || (y0 - m < 0) || (x1 + m > W) || (y1 + m > H);
     const int tw = x1 - x0, th = y1 - y0;
     const int tx = x0 - gx0, ty = y0 - gy0;  // tile origin in region coordinates
+    // The LDS window's corner is pixel (x0 - MMAX, y0 - MMAX) for every tile;
+    // (sx, sy) takes region coordinates to window coordinates.
+    const int wx0 = x0 - MMAX, wy0 = y0 - MMAX;
+    const int sx = gx0 - wx0, sy = gy0 - wy0;
+    // image borders the pixel at window cell (rx, ry) lies on: bit 0 left,
+    // 1 right, 2 top, 3 bottom
+    auto border = [&](int rx, int ry) {
+        const int gx = wx0 + rx, gy = wy0 + ry;
+        return (gx == 0) | (gx == W - 1) << 1 | (gy == 0) << 2 | (gy == H - 1) << 3;
+    };
 
-    // Every slot's region coordinates and its distance D outside the tile,
-    // packed D << 27 | ry << 16 | rx (-1: no pixel).  The region of reach e
-    // (the tile grown by e, clipped to the image) holds the slot exactly
-    // when D <= e, i.e. when (unsigned)pk < (e + 1) << 27: one compare.
+    // Every slot's window coordinates, its distance D outside the tile and
+    // its border flags E, packed D << 27 | ry << 16 | E << 12 | rx (-1: no
+    // pixel).  The region of reach e (the tile grown by e, clipped to the
+    // image) holds the slot exactly when D <= e, i.e. when (unsigned)pk <
+    // (e + 1) << 27: one compare.
     int pk[P];
 #pragma unroll
     for (int j = 0; j < PT; ++j) {
         const int i = threadIdx.x + j * NT;
         const int ly = i / TW, lx = i - ly * TW;
-        pk[j] = (lx < tw && ly < th) ? ((ty + ly) << 16) | (tx + lx) : -1;  // D = 0
+        const int rx = tx + lx + sx, ry = ty + ly + sy;
+        pk[j] = (lx < tw && ly < th) ? (ry << 16) | (border(rx, ry) << 12) | rx : -1;  // D = 0
     }
     {
         const int lw = tx, rw = gx1 - x1, sw = lw + rw;
@@ -177,7 +192,8 @@ __global__ __launch_bounds__(NT, 8) void hs_adj_tb_kernel(AdjTbArgs a, Taps taps
                 const int gx = gx0 + rx, gy = gy0 + ry;
                 const int dx = max(max(x0 - gx, gx - x1 + 1), 0);
                 const int dy = max(max(y0 - gy, gy - y1 + 1), 0);
-                pk[PT + j] = (max(dx, dy) << 27) | (ry << 16) | rx;
+                const int wx = rx + sx, wy = ry + sy;
+                pk[PT + j] = (max(dx, dy) << 27) | (wy << 16) | (border(wx, wy) << 12) | wx;
             }
         }
     }
@@ -195,8 +211,8 @@ __global__ __launch_bounds__(NT, 8) void hs_adj_tb_kernel(AdjTbArgs a, Taps taps
         g[j] = make_float2(0.0f, 0.0f);
         if (j < PT) ax[j] = ay[j] = at[j] = aa[j] = ct[j] = 0.0f;
         if (pk[j] < 0) continue;
-        const int rx = pk[j] & 0xffff, ry = (pk[j] >> 16) & 0x7ff;
-        const int64_t q = (int64_t)(gy0 + ry) * W + gx0 + rx;
+        const int rx = pk[j] & 0xfff, ry = (pk[j] >> 16) & 0x7ff;
+        const int64_t q = (int64_t)(wy0 + ry) * W + wx0 + rx;
         const float ix = Ix[q], iy = Iy[q];
         cx[j] = ix;
         cy[j] = iy;
@@ -228,7 +244,7 @@ __global__ __launch_bounds__(NT, 8) void hs_adj_tb_kernel(AdjTbArgs a, Taps taps
             lu[j] = lv[j] = 0.0f;
             if (pkj >= 0) {
                 const int64_t q =
-                    (int64_t)(gy0 + ((pkj >> 16) & 0x7ff)) * W + gx0 + (pkj & 0xffff);
+                    (int64_t)(wy0 + ((pkj >> 16) & 0x7ff)) * W + wx0 + (pkj & 0xfff);
                 lu[j] = U0[q];
                 lv[j] = V0[q];
             }
@@ -244,7 +260,7 @@ __global__ __launch_bounds__(NT, 8) void hs_adj_tb_kernel(AdjTbArgs a, Taps taps
             int pkj = pk[j];
             asm volatile("" : "+v"(pkj));
             if (!in_reach(pkj, e)) continue;
-            const int rx = pkj & 0xffff, ry = (pkj >> 16) & 0x7ff;
+            const int rx = pkj & 0xfff, ry = (pkj >> 16) & 0x7ff;
             // (fused multiply-adds: the adjoint is checked against fp64 and
             // finite differences, not bit for bit)
             const float s = -fmaf(cx[j], g[j].x, cy[j] * g[j].y) * cd[j];
@@ -260,63 +276,84 @@ __global__ __launch_bounds__(NT, 8) void hs_adj_tb_kernel(AdjTbArgs a, Taps taps
         }
     };
 
+    // Window cells outside the image read as zero in both buffers: pass B's
+    // unclamped gathers then take no term from a source that does not exist.
+    // (Pass A writes only image pixels, so the zeros stay.)
+    if (clipped) {
+        for (int c = threadIdx.x; c < NPX; c += NT) {
+            const int ry = c / PITCH, rx = c - ry * PITCH;
+            const int gx = wx0 + rx, gy = wy0 + ry;
+            if (gx < 0 || gx >= W || gy < 0 || gy >= H)
+                GB[0][c] = GB[1][c] = make_float2(0.0f, 0.0f);
+        }
+    }
+
     load_lin(1);
     pass_a(1, 0);
     __syncthreads();
 
     for (int step = 1; step <= m; ++step) {
         const int buf = (step - 1) & 1;
-        // pass B on R_{m-step}: g <- A^T g_b (gather over sources that clamp onto p)
+        // pass B on R_{m-step}: g <- A^T g_b.  A source q whose tap lands
+        // outside the image clamps onto the border pixel p: in the forward
+        // map that is a read of a ghost of p (the cell across the border,
+        // which holds p's value), so p also collects the unclamped gather at
+        // each of its ghosts (one per border it lies on, three at a corner).
+        // The zeros outside the image drop the sources that do not exist.
         const int e = m - step;
         const float2 *GBb = GB[buf];
+        auto gather = [&](const float2 *gb, float &au, float &av) {
+            if (NTAP > 0) {
+                constexpr int G = 4;
+#pragma unroll
+                for (int d0 = 0; d0 < NTAP; d0 += G) {
+                    float2 q[G];
+#pragma unroll
+                    for (int i = 0; i < G; ++i)
+                        if (d0 + i < NTAP) q[i] = gb[-tp.dl[d0 + i]];
+#pragma unroll
+                    for (int i = 0; i < G; ++i)
+                        if (d0 + i < NTAP) {
+                            au = fmaf(tp.w[d0 + i], q[i].x, au);
+                            av = fmaf(tp.w[d0 + i], q[i].y, av);
+                        }
+                    // the next group's reads stay behind this group's use
+                    asm volatile("" ::: "memory");
+                }
+            } else {
+#pragma unroll
+                for (int d = 0; d < HS_MAX_TAPS; ++d) {
+                    if (d < tp.nd) {
+                        const float2 q = gb[-tp.dl[d]];
+                        au = fmaf(tp.w[d], q.x, au);
+                        av = fmaf(tp.w[d], q.y, av);
+                    }
+                }
+            }
+        };
 #pragma unroll
         for (int j = 0; j < P; ++j) {
             int pkj = pk[j];
             asm volatile("" : "+v"(pkj));
             if (!in_reach(pkj, e)) continue;
-            const int rx = pkj & 0xffff, ry = (pkj >> 16) & 0x7ff;
-            const int gx = gx0 + rx, gy = gy0 + ry;
+            const int rx = pkj & 0xfff, ry = (pkj >> 16) & 0x7ff;
+            const float2 *gb = GBb + ry * PITCH + rx;
             float au = 0.0f, av = 0.0f;
-            if (!clipped || (gx > 0 && gx < W - 1 && gy > 0 && gy < H - 1)) {
-                const float2 *gb = GBb + ry * PITCH + rx;
-                if (NTAP > 0) {
-                    constexpr int G = 4;
-#pragma unroll
-                    for (int d0 = 0; d0 < NTAP; d0 += G) {
-                        float2 q[G];
-#pragma unroll
-                        for (int i = 0; i < G; ++i)
-                            if (d0 + i < NTAP) q[i] = gb[-tp.dl[d0 + i]];
-#pragma unroll
-                        for (int i = 0; i < G; ++i)
-                            if (d0 + i < NTAP) {
-                                au = fmaf(tp.w[d0 + i], q[i].x, au);
-                                av = fmaf(tp.w[d0 + i], q[i].y, av);
-                            }
-                        // the next group's reads stay behind this group's use
-                        asm volatile("" ::: "memory");
-                    }
-                } else {
-#pragma unroll
-                    for (int d = 0; d < HS_MAX_TAPS; ++d) {
-                        if (d < tp.nd) {
-                            const float2 q = gb[-tp.dl[d]];
-                            au = fmaf(tp.w[d], q.x, au);
-                            av = fmaf(tp.w[d], q.y, av);
-                        }
-                    }
+            gather(gb, au, av);
+            const int eb = (pkj >> 12) & 15;
+            if (clipped && eb) {
+                const bool l = eb & 1, r = eb & 2;
+                if (l) gather(gb - 1, au, av);
+                if (r) gather(gb + 1, au, av);
+                if (eb & 4) {
+                    gather(gb - PITCH, au, av);
+                    if (l) gather(gb - PITCH - 1, au, av);
+                    if (r) gather(gb - PITCH + 1, au, av);
                 }
-            } else {
-                for (int d = 0; d < taps.nd; ++d) {
-                    int lox, hix, loy, hiy;
-                    tr_range(gx, taps.ox[d], W, lox, hix);
-                    tr_range(gy, taps.oy[d], H, loy, hiy);
-                    for (int qy = loy; qy <= hiy; ++qy)
-                        for (int qx = lox; qx <= hix; ++qx) {
-                            const float2 q = GBb[(qy - gy0) * PITCH + (qx - gx0)];
-                            au = fmaf(taps.w[d], q.x, au);
-                            av = fmaf(taps.w[d], q.y, av);
-                        }
+                if (eb & 8) {
+                    gather(gb + PITCH, au, av);
+                    if (l) gather(gb + PITCH - 1, au, av);
+                    if (r) gather(gb + PITCH + 1, au, av);
                 }
             }
             g[j] = make_float2(au, av);
@@ -334,8 +371,8 @@ __global__ __launch_bounds__(NT, 8) void hs_adj_tb_kernel(AdjTbArgs a, Taps taps
 #pragma unroll
     for (int j = 0; j < PT; ++j) {
         if (pk[j] < 0) continue;
-        const int rx = pk[j] & 0xffff, ry = (pk[j] >> 16) & 0x7ff;
-        const int64_t q = off + (int64_t)(gy0 + ry) * W + gx0 + rx;
+        const int rx = pk[j] & 0xfff, ry = (pk[j] >> 16) & 0x7ff;
+        const int64_t q = off + (int64_t)(wy0 + ry) * W + wx0 + rx;
         a.gu_out[q] = g[j].x;
         a.gv_out[q] = g[j].y;
         a.gIx[q] = ax[j];

@@ -16,13 +16,17 @@
 // A^T of a clamped stencil is not the stencil: a border pixel receives every
 // tap that clamps onto it.  Along one axis, clamp(q + o) = p holds for q in
 //   [p == 0 ? 0 : p - o,  p == n-1 ? n-1 : p - o]  intersected with [0, n-1],
-// so the transpose is a gather over at most (|o|+1)^2 sources per tap.
+// so the transpose is a gather over at most (|o|+1)^2 sources per tap.  For a
+// radius-1 stencil the same sum is the unclamped gather at p plus the
+// unclamped gathers at the ghosts of p (the cells just across each image border
+// p lies on, whose forward value is p's), with zero outside the image: the
+// tiled reverse sweep computes it that way.
 //
 // Memory: states are checkpointed every C iterations in the forward pass; the
 // reverse pass recomputes one segment of C steps at a time with the same fused
 // kernel (bit-identical states), keeping for each step only what its reverse
-// step needs on the pixel itself -- t and (ubar, vbar) -- then walks the
-// segment backwards.
+// step needs on the pixel itself -- (ubar, vbar), from which t is recomputed --
+// then walks the segment backwards.
 #include "hmx_common.h"
 #include "hs_tile.h"
 
@@ -79,9 +83,10 @@ __device__ inline float tap_apply_T(const float *__restrict__ g, int H, int W, i
 
 // ---------------------------------------------------------------------------
 // Reverse sweep, temporally blocked: one launch walks m (<= MMAX) reverse
-// steps for an output tile.  The gradient (gu, gv) lives in LDS on the tile
-// grown by m (clipped to the image) and shrinks by one pixel per step, exactly
-// like the forward kernel's region; the accumulators gIx, gIy, gIt of the
+// steps for an output tile.  The gradient (gu, gv) lives on the tile grown by
+// m (clipped to the image) and shrinks by one pixel per step, exactly like the
+// forward kernel's region, inside an LDS window at a fixed offset from the
+// tile whose cells outside the image hold zero; the accumulators gIx, gIy, gIt of the
 // tile's own pixels stay in registers and are stored once per launch; each
 // step stages only the recomputed (ubar, vbar) of its forward step on the
 // tile itself.

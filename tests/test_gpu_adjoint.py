@@ -74,6 +74,27 @@ def test_small_vs_fp64_oracle_gradient():
         assert abs(ga[b] - ra) <= 1e-3 * abs(ra)
 
 
+@pytest.mark.parametrize("shape", [(1, 9), (9, 1), (2, 3), (3, 70), (33, 65)])
+def test_border_heavy_shapes_vs_fp64_oracle_gradient(shape):
+    """Images where most pixels lie on a border (a pixel on both the left and the
+    right border when W = 1, on top and bottom when H <= 2): the reverse sweep's
+    transposed clamp comes from its ghost gathers.  12 steps in checkpoint
+    segments of 5, so reverse launches of fewer than 8 steps are covered too."""
+    H, W = shape
+    B, n = 2, 12
+    I1, I2, fx, fy, alpha = batch(B, H, W, 4)
+    gI1, gI2, ga, u, v = run_grad(I1, I2, fx, fy, alpha, n, 5)
+    scale = 2.0 / (B * H * W)
+    for b in range(B):
+        a64 = float(alpha[b])
+        Ix, Iy, It = oracle.hs_setup(I1[b], I2[b], TAPS["Dx"], TAPS["Dy"], TAPS["Dt"], np.float64)
+        u64, v64 = oracle.hs_jacobi(Ix, Iy, It, a64 * a64, TAPS["Avg"], n, dtype=np.float64)
+        r1, r2, ra, _, _ = oracle.hs_grad(I1[b], I2[b], a64, TAPS, n, scale * (u64 - fx[b]),
+                                          scale * (v64 - fy[b]))
+        assert relerr(gI1[b], r1) < 1e-3 and relerr(gI2[b], r2) < 1e-3, (shape, b)
+        assert abs(ga[b] - ra) <= 1e-3 * abs(ra), (shape, b)
+
+
 def test_checkpoint_interval_invariance():
     I1, I2, fx, fy, alpha = batch(2, 33, 47, 3)
     outs = [run_grad(I1, I2, fx, fy, alpha, 25, c) for c in (1, 6, 25, 40)]

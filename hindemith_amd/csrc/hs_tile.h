@@ -199,6 +199,8 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
                 pk[j] = ((k - max(dx, dy)) << 27) | (ry << 16) | (e << 12) | rx;
                 const int64_t g = (int64_t)gy * a.W + gx;
                 const float ix = a.Ix[g], iy = a.Iy[g];
+                // (u, v) issued with the coefficients: one wait per cell
+                const float u_in = a.u_in[g], v_in = a.v_in[g];
                 cx[j] = ix;
                 cy[j] = iy;
                 ct[j] = a.It[g];
@@ -211,7 +213,7 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
                     cd[j] = s1 + m2;
                 }
                 if (FDIV) cr[j] = hs_recip(cd[j]);
-                uv[c] = make_float2(a.u_in[g], a.v_in[g]);
+                uv[c] = make_float2(u_in, v_in);
             } else {
                 const int64_t g =
                     (int64_t)clampi(gy, 0, a.H - 1) * a.W + clampi(gx, 0, a.W - 1);

@@ -206,7 +206,8 @@ int hmx_hs_variant_info(int variant, int *tile_w, int *tile_h, int *threads, int
  * rectangular-tile variant (hs_tb_kernel), or a negative HMX_ERR_* code. */
 int hmx_hs_variant_strip(int variant);
 /* Time every (variant, k) candidate on the given buffers (their contents are
- * overwritten) and cache the best for this H x W; returns the choice. */
+ * overwritten) and cache the best for this H x W; returns the choice.  The
+ * default configuration is kept unless a candidate is more than 2% faster. */
 int hmx_hs_tune(const float *Ix, const float *Iy, const float *It, float alpha2, int H, int W,
                 int nda, const float *wa, const int *oxa, const int *oya, float *u, float *v,
                 float *work_u, float *work_v, void *stream, int *variant, int *k);

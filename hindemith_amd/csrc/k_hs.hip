@@ -2,7 +2,7 @@
 //
 // One launch advances the flow field by k iterations.  A work-group owns a
 // TW x TH output tile.  It loads the tile grown by k pixels on each side
-// (clipped to the image) once: u and v interleaved as float2 in LDS, and Ix,
+// once (ghosts outside the image, below): u and v interleaved as float2 in LDS, and Ix,
 // Iy, It plus den = alpha2 + Ix*Ix + Iy*Iy in registers of the thread that
 // owns each pixel.  It then runs k Jacobi steps entirely on chip; step j
 // updates the tile grown by k-j pixels, which is exactly the set whose
@@ -16,9 +16,10 @@
 // with den = (alpha2 + Ix*Ix) + Iy*Iy, so the result is bit-identical to
 // running the reference's per-op kernels with that rounding order.
 //
-// Replicated borders (Stencil2DMVM.cpp:42-45 clamp): a tile whose grown
-// region was clipped by the image uses clamped LDS coordinates; all others use
-// precomputed tap deltas.
+// Replicated borders (Stencil2DMVM.cpp:42-45 clamp): where a tile's grown
+// region reaches past the image, its cells outside the image are ghosts that
+// hold the pixel they clamp to (hs_tile.h), so every tile reads its taps at
+// the same precomputed deltas.
 #include "hs_tile.h"
 
 #include <algorithm>

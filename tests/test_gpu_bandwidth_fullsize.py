@@ -1,0 +1,119 @@
+"""Parity of the bandwidth kernels on the benchmark's 2048 x 2048 grid, on the
+inputs the benchmark times (x ~ U(-1, 1) with seed 0; the power-law matrix
+from seed 1).
+
+* 5-point Laplacian as a Stencil apply (Stencil2DMVM.cpp:36-47): bit-exact.
+* The same operator as SpMat2D planes (SpmvDIA2D.cpp:33-56), and as a 1-D
+  dia_matrix over the flattened grid (SpmvDIA.cpp:30-50): bit-exact; also with
+  random planes, where reading the coefficient at the wrong index shows.
+* The same operator as CSR (SpmvCSR.cpp:30-47): 4,194,304 rows; one lane per
+  row is the reference's sequential fma chain, bit-exact; the lane heuristic's
+  choice within the multi-lane bound of the fp64 product.
+* A Zipf-1.5 power-law CSR (4,194,304 rows, 1-512 entries per row, ~72.8 M
+  entries) through the row-block, row-binned and one-sub-wave-per-row launches:
+  within (len + 2) * eps32 * sum |a||x| of the fp64 product per row, and the
+  row-block result bit-identical on a second call.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from hindemith_amd import synthetic as S
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+H = W = 2048
+N = H * W
+EPS = np.finfo(np.float32).eps
+
+
+@pytest.fixture(scope="module")
+def K():
+    from hindemith_amd import kernels
+    return kernels
+
+
+@pytest.fixture(scope="module")
+def x_host():
+    return np.random.default_rng(0).uniform(-1, 1, (H, W)).astype(np.float32)
+
+
+def _host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def _within_csr_bound(y, indptr, indices, data, x):
+    ref = oracle.csr_spmv(indptr, indices, data, x, dtype=np.float64)
+    absdot = oracle.csr_spmv(indptr, indices, np.abs(data), np.abs(x), dtype=np.float64)
+    lens = np.diff(indptr).astype(np.float64)
+    err = np.abs(y.astype(np.float64) - ref)
+    bound = (lens + 2) * EPS * absdot + 1e-30
+    return bool(np.all(err <= bound)), float(np.max(err / bound))
+
+
+def test_stencil_5pt_2048(K, x_host):
+    lap = S.laplacian5_taps()
+    y = _host(K.stencil2d(torch.from_numpy(x_host).cuda(), lap))
+    assert np.array_equal(y, oracle.stencil2d(x_host, lap))
+
+
+@pytest.mark.parametrize("planes_kind", ["laplacian", "random"])
+def test_dia2d_5plane_2048(K, x_host, planes_kind):
+    planes, ox, oy = S.laplacian5_dia_planes(H, W)
+    if planes_kind == "random":
+        planes = np.random.default_rng(2).normal(size=planes.shape).astype(np.float32)
+    y = _host(K.dia2d_spmv(torch.from_numpy(planes).cuda(), ox, oy,
+                           torch.from_numpy(x_host).cuda()))
+    assert np.array_equal(y, oracle.dia2d(planes, ox, oy, x_host))
+
+
+def test_dia1d_5diag_2048(K, x_host):
+    planes, _, _ = S.laplacian5_dia_planes(H, W)
+    off1 = np.array([0, -1, 1, -W, W], np.int32)
+    data = np.ascontiguousarray(planes.reshape(5, N))
+    xf = x_host.reshape(-1)
+    y = _host(K.dia_spmv(torch.from_numpy(data).cuda(), off1, torch.from_numpy(xf).cuda()))
+    assert np.array_equal(y, oracle.dia1d(data, off1, xf))
+
+
+@pytest.mark.parametrize("lanes", [1, 0])
+def test_csr_5pt_2048(K, x_host, lanes):
+    planes, ox, oy = S.laplacian5_dia_planes(H, W)
+    ip, ix, d = S.dia2d_to_csr(planes, ox, oy)
+    assert len(ip) - 1 == N and int(ip[-1]) == 20_963_328
+    xf = x_host.reshape(-1)
+    y = _host(K.csr_spmv(*(torch.from_numpy(a).cuda() for a in (ip, ix, d, xf)),
+                         lanes_per_row=lanes))
+    if lanes == 1:
+        assert np.array_equal(y, oracle.csr_spmv(ip, ix, d, xf))
+    ok, worst = _within_csr_bound(y, ip, ix, d, xf)
+    assert ok, worst
+
+
+@pytest.fixture(scope="module")
+def powerlaw():
+    ip, ix, d = S.powerlaw_csr(N, np.random.default_rng(1))
+    return ip, ix, d
+
+
+@pytest.mark.parametrize("form", ["rowblocks", "binned", "unbinned"])
+def test_csr_powerlaw_2048(K, x_host, powerlaw, form):
+    ip, ix, d = powerlaw
+    nnz = int(ip[-1])
+    assert 70_000_000 < nnz < 76_000_000, nnz
+    xf = x_host.reshape(-1)
+    tip, tix, td, tx = (torch.from_numpy(a).cuda() for a in (ip, ix, d, xf))
+    if form == "rowblocks":
+        rb = K.CSRRowBlocks(ip, "cuda")
+        y = _host(K.csr_spmv_rowblocks(tip, tix, td, tx, rb))
+        y2 = _host(K.csr_spmv_rowblocks(tip, tix, td, tx, rb))
+        assert y.tobytes() == y2.tobytes()
+    elif form == "binned":
+        y = _host(K.csr_spmv_binned(tip, tix, td, tx, K.CSRBins(ip, "cuda")))
+    else:
+        y = _host(K.csr_spmv(tip, tix, td, tx))
+    del tip, tix, td, tx
+    ok, worst = _within_csr_bound(y, ip, ix, d, xf)
+    assert ok, (form, worst)

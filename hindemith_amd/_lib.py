@@ -49,6 +49,10 @@ _SIGS = {
     "hmx_csr_spmv_binned": [c_int, vp, vp, vp, vp, vp, vp, vp, vp],
     "hmx_csr_row_blocks": [c_int, vp, vp, vp],
     "hmx_csr_spmv_rowblocks": [c_int, vp, vp, vp, vp, vp, vp, c_int, vp],
+    "hmx_csr_slice_count": [],
+    "hmx_csr_slice_build": [c_int, c_int, vp, vp, vp, vp, vp],
+    "hmx_csr_slice_values": [c_int64, vp, vp, vp, vp],
+    "hmx_csr_spmv_sliced": [c_int, vp, vp, vp, vp, vp, vp, vp],
     "hmx_sum2d": [vp, c_int64, vp, vp, vp],
     "hmx_hs_setup": [vp, vp, c_int, c_int] + [c_int, vp, vp, vp] * 3 + [vp, vp, vp, vp],
     "hmx_hs_jacobi": [vp, vp, vp, vp, c_float, c_int, c_int, c_int, vp, vp, vp, c_int,

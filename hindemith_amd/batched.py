@@ -1,12 +1,14 @@
-"""Many Horn-Schunck pairs of uneven sizes in one persistent launch.
+"""Many Horn-Schunck pairs of uneven sizes in one batched call.
 
 ``BatchedHS(I1s, I2s, alphas, iters)`` keeps per-pair device buffers and a
-ctypes array of ``hmx_pair`` descriptors; ``run()`` issues ONE launch
-(``hmx_hs_batched``) that does the derivative setup and all Jacobi
-iterations of every pair, captured in a hipGraph on first use and replayed
-afterwards.  ``tune()`` picks the tile shape and iterations per barrier phase
-for this list of sizes.  ``solve_each()`` is the plain alternative: one
-``hmx_hs_solve`` per pair.
+ctypes array of ``hmx_pair`` descriptors; ``run()`` makes one call of
+``hmx_hs_batched``, which does the derivative setup and all Jacobi
+iterations of every pair over one tile list of all pairs, captured in a
+hipGraph on first use and replayed afterwards.  The default (stepped) form is
+one launch per phase of k iterations over every pair's tiles; the persistent
+form is one launch with a grid barrier between phases.  ``tune()`` picks the
+form, tile shape and iterations per phase for this list of sizes.
+``solve_each()`` is the plain alternative: one ``hmx_hs_solve`` per pair.
 """
 from __future__ import annotations
 
@@ -99,7 +101,9 @@ class BatchedHS:
             t.zero_()
 
     def run(self, graph=True, sync=False, spin_limit=0, variant=None, k=None):
-        """One persistent launch over all pairs; returns the list of (u, v)."""
+        """One batched call over all pairs; returns the list of (u, v).  A
+        persistent variant always synchronises and raises if its grid barrier
+        gave up; sync=True makes the stepped form synchronise too."""
         self.reset()
         flags = (BATCH_GRAPH if graph else 0) | (BATCH_SYNC if sync else 0)
         L = _sig()

@@ -60,11 +60,16 @@ __host__ __device__ inline int clampi(int v, int lo, int hi) {
 // blockIdx -> logical tile index so that the tiles one XCD receives (blocks
 // are dealt round-robin over the 8 XCDs) are spatially contiguous, which lets
 // neighbouring tiles share halo lines in that XCD's L2.
-__device__ inline int xcd_remap(int b, int n) {
+// rev: each XCD walks its range from the far end instead.  Launches that
+// alternate rev begin with the tiles the previous launch ended with, whose
+// data that XCD's L2 still holds.
+__device__ inline int xcd_remap(int b, int n, bool rev = false) {
     constexpr int NX = 8;
     int per = n / NX, rem = n % NX;
     int x = b % NX, slot = b / NX;
-    return (x < rem) ? x * (per + 1) + slot : rem * (per + 1) + (x - rem) * per + slot;
+    const int cnt = per + (x < rem ? 1 : 0);
+    const int base = (x < rem) ? x * (per + 1) : rem * (per + 1) + (x - rem) * per;
+    return base + (rev ? cnt - 1 - slot : slot);
 }
 
 // Device-side helpers used by several kernels.

@@ -181,44 +181,123 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
     // Ghosts and cells outside the region hold -1.
     float cx[P], cy[P], ct[P], cd[P], cr[FDIV ? P : 1];
     int pk[P];
+    if (!clipped) {
+        // Unclipped tile (most of them): the region is the window's top-left
+        // (TW + 2k) x (TH + 2k) cells and every one of them is an image pixel,
+        // so a cell's offset, last step and border flags (none) follow from
+        // its window position alone.  All of a thread's loads are issued
+        // before any of them is used.
+        // Byte offsets from the region's corner fit 32 bits (H, W < 32768),
+        // so each load is a uniform base plus one per-cell offset register.
+        const int64_t corner = ((int64_t)gy0 * a.W + gx0) * (int64_t)sizeof(float);
+        const char *const pIx = reinterpret_cast<const char *>(a.Ix) + corner;
+        const char *const pIy = reinterpret_cast<const char *>(a.Iy) + corner;
+        const char *const pIt = reinterpret_cast<const char *>(a.It) + corner;
+        const char *const pu = reinterpret_cast<const char *>(a.u_in) + corner;
+        const char *const pv = reinterpret_cast<const char *>(a.v_in) + corner;
+        const char *const pd =
+            a.den ? reinterpret_cast<const char *>(a.den) + corner : nullptr;
+        const unsigned rowb = (unsigned)a.W * (unsigned)sizeof(float);
+        float ui[P], vi[P];
+        bool in[P];
+        int rxs[P], rys[P];
 #pragma unroll
-    for (int j = 0; j < P; ++j) {
-        const int c = tid + j * NT;
-        if (FDIV) cr[j] = 1.0f;
-        const int ry = c / PITCH, rx = c - ry * PITCH;
-        pk[j] = -1;
-        cx[j] = cy[j] = ct[j] = 0.0f;
-        cd[j] = 1.0f;
-        if (c < NPX && rx < RW && ry < RH) {
+        for (int j = 0; j < P; ++j) {
+            const int c = tid + j * NT;
+            const int ry = c / PITCH, rx = c - ry * PITCH;
+            rxs[j] = rx;
+            rys[j] = ry;
+            in[j] = ((j + 1) * NT <= NPX || c < NPX) && rx < RW && ry < RH;
+            // a cell outside the region is never active (pk < 0): its
+            // registers are left unset
+            if (in[j]) {
+                const uint64_t o = (unsigned)ry * rowb + (unsigned)rx * (unsigned)sizeof(float);
+                cx[j] = *reinterpret_cast<const float *>(pIx + o);
+                cy[j] = *reinterpret_cast<const float *>(pIy + o);
+                ct[j] = *reinterpret_cast<const float *>(pIt + o);
+                ui[j] = *reinterpret_cast<const float *>(pu + o);
+                vi[j] = *reinterpret_cast<const float *>(pv + o);
+                if (pd) cd[j] = *reinterpret_cast<const float *>(pd + o);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            if (FDIV) cr[j] = 1.0f;
+            pk[j] = -1;
+            if (in[j]) {
+                const int rx = rxs[j], ry = rys[j];
+                const int dx = max(max(k - rx, rx - (k + TW - 1)), 0);
+                const int dy = max(max(k - ry, ry - (k + TH - 1)), 0);
+                pk[j] = ((k - max(dx, dy)) << 27) | (ry << 16) | rx;
+                if (!pd) {
+                    const float m1 = cx[j] * cx[j];
+                    const float s1 = a.alpha2 + m1;
+                    const float m2 = cy[j] * cy[j];
+                    cd[j] = s1 + m2;
+                }
+                if (FDIV) cr[j] = hs_recip(cd[j]);
+                uv[tid + j * NT] = make_float2(ui[j], vi[j]);
+            }
+        }
+    } else {
+        // Clipped tile: region cells outside the image are ghosts and load the
+        // pixel their coordinates clamp to; image cells also load their
+        // coefficients.  Byte offsets from the image start fit 32 bits
+        // (H, W < 32768).  As above, every load is issued before any use.
+        const char *const bIx = reinterpret_cast<const char *>(a.Ix);
+        const char *const bIy = reinterpret_cast<const char *>(a.Iy);
+        const char *const bIt = reinterpret_cast<const char *>(a.It);
+        const char *const bu = reinterpret_cast<const char *>(a.u_in);
+        const char *const bv = reinterpret_cast<const char *>(a.v_in);
+        const char *const bd = reinterpret_cast<const char *>(a.den);
+        const unsigned rowb = (unsigned)a.W * (unsigned)sizeof(float);
+        float ui[P], vi[P];
+        bool in[P], img[P];
+        int gxs[P], gys[P], rxs[P], rys[P];
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            const int c = tid + j * NT;
+            const int ry = c / PITCH, rx = c - ry * PITCH;
             const int gx = gx0 + rx, gy = gy0 + ry;
-            if (gx >= 0 && gx < a.W && gy >= 0 && gy < a.H) {
+            rxs[j] = rx;
+            rys[j] = ry;
+            gxs[j] = gx;
+            gys[j] = gy;
+            in[j] = ((j + 1) * NT <= NPX || c < NPX) && rx < RW && ry < RH;
+            img[j] = in[j] && gx >= 0 && gx < a.W && gy >= 0 && gy < a.H;
+            const unsigned o = (unsigned)clampi(gy, 0, a.H - 1) * rowb +
+                               (unsigned)clampi(gx, 0, a.W - 1) * (unsigned)sizeof(float);
+            if (in[j]) {
+                ui[j] = *reinterpret_cast<const float *>(bu + o);
+                vi[j] = *reinterpret_cast<const float *>(bv + o);
+            }
+            if (img[j]) {
+                cx[j] = *reinterpret_cast<const float *>(bIx + o);
+                cy[j] = *reinterpret_cast<const float *>(bIy + o);
+                ct[j] = *reinterpret_cast<const float *>(bIt + o);
+                if (bd) cd[j] = *reinterpret_cast<const float *>(bd + o);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            if (FDIV) cr[j] = 1.0f;
+            pk[j] = -1;
+            if (img[j]) {
+                const int gx = gxs[j], gy = gys[j];
                 const int dx = max(max(x0 - gx, gx - x1 + 1), 0);
                 const int dy = max(max(y0 - gy, gy - y1 + 1), 0);
                 const int e = (gx == 0) | (gx == a.W - 1) << 1 | (gy == 0) << 2 |
                               (gy == a.H - 1) << 3;
-                pk[j] = ((k - max(dx, dy)) << 27) | (ry << 16) | (e << 12) | rx;
-                const int64_t g = (int64_t)gy * a.W + gx;
-                const float ix = a.Ix[g], iy = a.Iy[g];
-                // (u, v) issued with the coefficients: one wait per cell
-                const float u_in = a.u_in[g], v_in = a.v_in[g];
-                cx[j] = ix;
-                cy[j] = iy;
-                ct[j] = a.It[g];
-                if (a.den) {
-                    cd[j] = a.den[g];
-                } else {
-                    const float m1 = ix * ix;
+                pk[j] = ((k - max(dx, dy)) << 27) | (rys[j] << 16) | (e << 12) | rxs[j];
+                if (!bd) {
+                    const float m1 = cx[j] * cx[j];
                     const float s1 = a.alpha2 + m1;
-                    const float m2 = iy * iy;
+                    const float m2 = cy[j] * cy[j];
                     cd[j] = s1 + m2;
                 }
                 if (FDIV) cr[j] = hs_recip(cd[j]);
-                uv[c] = make_float2(u_in, v_in);
-            } else {
-                const int64_t g =
-                    (int64_t)clampi(gy, 0, a.H - 1) * a.W + clampi(gx, 0, a.W - 1);
-                uv[c] = make_float2(a.u_in[g], a.v_in[g]);
             }
+            if (in[j]) uv[tid + j * NT] = make_float2(ui[j], vi[j]);
         }
     }
     __syncthreads();

@@ -42,11 +42,11 @@ int launch_hs_tb(int variant, const float *Ix, const float *Iy, const float *It,
                  const float *den, float alpha2, int H, int W, const Taps &taps, int k,
                  const float *u_in, const float *v_in, float *u_out, float *v_out,
                  hipStream_t s, int B, const float *alpha2_b, float *lt, float *lub,
-                 float *lvb, int64_t lstride);
+                 float *lvb, int64_t lstride, bool rev);
 int hs_jacobi_run(const float *Ix, const float *Iy, const float *It, const float *den,
                   float alpha2, int H, int W, const Taps &taps, int iters, float *u, float *v,
                   float *wu, float *wv, int variant, int k, hipStream_t s, int B,
-                  const float *alpha2_b);
+                  const float *alpha2_b, bool start_in_work);
 void hs_choice(int H, int W, int &variant, int &k);
 
 __device__ inline void tr_range(int p, int o, int n, int &lo, int &hi) {
@@ -450,7 +450,7 @@ __global__ void galpha_finish_kernel(const double *__restrict__ partial, const f
 // Batched setup: one launch per pair of the single-pair kernel.
 int launch_hs_setup(const float *I1, const float *I2, int H, int W, const Taps &dx,
                     const Taps &dy, const Taps &dt, float *Ix, float *Iy, float *It,
-                    hipStream_t s);
+                    hipStream_t s, const float *u, const float *v, float *wu, float *wv);
 
 static int setup_batch(const float *I1, const float *I2, int B, int H, int W, const Taps &dx,
                        const Taps &dy, const Taps &dt, float *Ix, float *Iy, float *It,
@@ -458,7 +458,7 @@ static int setup_batch(const float *I1, const float *I2, int B, int H, int W, co
     const int64_t n = (int64_t)H * W;
     for (int b = 0; b < B; ++b) {
         int rc = launch_hs_setup(I1 + b * n, I2 + b * n, H, W, dx, dy, dt, Ix + b * n,
-                                 Iy + b * n, It + b * n, s);
+                                 Iy + b * n, It + b * n, s, nullptr, nullptr, nullptr, nullptr);
         if (rc) return rc;
     }
     return HMX_OK;
@@ -549,7 +549,7 @@ extern "C" int hmx_hs_forward_ckpt(const float *I1, const float *I2, int B, int 
         HMX_HIP_TRY(hipMemcpyAsync(cv, v, n * 4, hipMemcpyDeviceToDevice, s));
         const int len = (iters - c * ckpt_every) < ckpt_every ? iters - c * ckpt_every : ckpt_every;
         rc = hs_jacobi_run(Ix, Iy, It, nullptr, 0.0f, H, W, T.av, len, u, v, wu, wv, variant, k, s,
-                           B, alpha2);
+                           B, alpha2, false);
         if (rc) return rc;
     }
     return HMX_OK;
@@ -627,7 +627,7 @@ extern "C" int hmx_hs_adjoint(const float *I1, const float *I2, int B, int H, in
             rc = launch_hs_tb(ADJ_RECOMPUTE_VARIANT, Ix, Iy, It, nullptr, 0.0f, H, W, T.av, kk,
                               j0 == 0 ? cu : rs[rcur][0], j0 == 0 ? cv : rs[rcur][1],
                               rs[rcur ^ 1][0], rs[rcur ^ 1][1], s, B, alpha2, nullptr, lub,
-                              lub + n, 2 * n);
+                              lub + n, 2 * n, (j0 / ADJ_RECOMPUTE_K) & 1);
             if (rc) return rc;
             rcur ^= 1;
             j0 += kk;

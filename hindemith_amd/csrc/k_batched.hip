@@ -11,7 +11,9 @@
 // then ceil(iters / k) phases of k fused Jacobi steps (the same tile body as
 // the single-image kernel, hs_tile.h), with a grid-wide barrier between
 // phases so every tile of every pair sees the previous phase's halo.  Pairs
-// ping-pong between (u, v) and (wu, wv); an odd phase count ends with a copy.
+// ping-pong between (u, v) and (wu, wv) so that the last phase writes (u, v):
+// with an odd phase count the setup also copies (u, v) into (wu, wv) and the
+// first phase reads from there.
 //
 // The barrier is an arrival counter in device memory: each work-group's
 // first thread releases its writes (agent-scope fence), increments the
@@ -48,6 +50,7 @@ struct BatchArgs {
     int *status;
     long long spin_limit;
     int npairs, total_tiles, iters, k;
+    int phase0;  // 1: odd phase count, phase 0 reads (wu, wv) (see phase_image)
 };
 
 struct SetupTaps {
@@ -132,6 +135,10 @@ __device__ __forceinline__ void setup_tile(const BatchArgs &b, const SetupTaps &
         const int r = tyi * TH + i / TW, c = txi * TW + i % TW;
         if (r >= P.H || c >= P.W) continue;
         const int64_t p = (int64_t)r * P.W + c;
+        if (b.phase0) {  // the first phase reads the start state from (wu, wv)
+            P.wu[p] = P.u[p];
+            P.wv[p] = P.v[p];
+        }
         if (TAP4) {
             const float a1 = setup_tap4(P.I1, P.H, P.W, r, c, st.dx);
             const float a2 = setup_tap4(P.I2, P.H, P.W, r, c, st.dx);
@@ -156,30 +163,15 @@ __device__ __forceinline__ void setup_tile(const BatchArgs &b, const SetupTaps &
     }
 }
 
-// (u, v) <- (wu, wv) on tile w (after an odd number of phases).
-template <int NT, int TW, int TH>
-__device__ __forceinline__ void copy_tile(const BatchArgs &b, int w) {
-    const PairDesc &P = b.pairs[b.tile_pair[w]];
-    const int t = w - P.tile_base;
-    const int tyi = t / P.tiles_x, txi = t - tyi * P.tiles_x;
-    for (int i = threadIdx.x; i < TW * TH; i += NT) {
-        const int r = tyi * TH + i / TW, c = txi * TW + i % TW;
-        if (r >= P.H || c >= P.W) continue;
-        const int64_t p = (int64_t)r * P.W + c;
-        P.u[p] = P.wu[p];
-        P.v[p] = P.wv[p];
-    }
-}
-
-// The image a phase reads and writes: even phases (u, v) -> (wu, wv), odd
-// phases back.
-__device__ __forceinline__ HsImage phase_image(const PairDesc &P, int ph) {
+// The image phase ph reads and writes: (u, v) -> (wu, wv) when ph + phase0 is
+// even, back otherwise, so the last phase always writes (u, v).
+__device__ __forceinline__ HsImage phase_image(const PairDesc &P, int ph, int phase0) {
     HsImage im;
     im.Ix = P.Ix;
     im.Iy = P.Iy;
     im.It = P.It;
     im.den = nullptr;
-    const bool even = (ph & 1) == 0;
+    const bool even = ((ph + phase0) & 1) == 0;
     im.u_in = even ? P.u : P.wu;
     im.v_in = even ? P.v : P.wv;
     im.u_out = even ? P.wu : P.u;
@@ -230,20 +222,7 @@ __global__ __launch_bounds__(NT, WPE) void hs_batched_kernel(BatchArgs b, SetupT
             if (threadIdx.x == 0) next = atomicAdd(ctr, 1u);  // overlaps this tile
             const int w = base + i;
             const PairDesc &P = b.pairs[b.tile_pair[w]];
-            HsImage im;
-            im.Ix = P.Ix;
-            im.Iy = P.Iy;
-            im.It = P.It;
-            im.den = nullptr;
-            const bool even = (ph & 1) == 0;
-            im.u_in = even ? P.u : P.wu;
-            im.v_in = even ? P.v : P.wv;
-            im.u_out = even ? P.wu : P.u;
-            im.v_out = even ? P.wv : P.v;
-            im.alpha2 = P.alpha2;
-            im.H = P.H;
-            im.W = P.W;
-            im.tiles_x = P.tiles_x;
+            const HsImage im = phase_image(P, ph, b.phase0);
             // per-cell tap addresses: the persistent kernel's own state leaves
             // no registers for the shared ones
             hs_tile<NT, TW, TH, KMAX, NTAP, false, DB, FDIV, false, false>(im, w - P.tile_base, kk, tp, uv);
@@ -251,9 +230,6 @@ __global__ __launch_bounds__(NT, WPE) void hs_batched_kernel(BatchArgs b, SetupT
         arrivals += gridDim.x;
         if (!grid_sync(b.barrier, arrivals, b.status, b.spin_limit)) return;
     }
-
-    if (nph & 1)  // the last phase wrote (wu, wv)
-        for (int i = slot; i < cnt; i += per_xcd) copy_tile<NT, TW, TH>(b, base + i);
 }
 
 // ---------------------------------------------------------------------------
@@ -261,7 +237,7 @@ __global__ __launch_bounds__(NT, WPE) void hs_batched_kernel(BatchArgs b, SetupT
 
 // ---------------------------------------------------------------------------
 // Stepped form: the same tile list, one ordinary launch per phase (plus one
-// for the setup and, after an odd number of phases, one copy).  Kernel
+// for the setup).  Kernel
 // boundaries order the phases, so there is no grid barrier and no residency
 // limit, and the tile body is the single-image kernel's full form (shared tap
 // addresses, read-ahead, reciprocal quotient).  One work-group per tile; the
@@ -282,48 +258,44 @@ __global__ __launch_bounds__(NT) void hs_multi_phase_kernel(BatchArgs b, Taps ta
     using Win = HsWindow<NT, TW, TH, KMAX, DB>;
     __shared__ float2 uvm[Win::SIZE];
     const HsTapTable<TW + 2 * KMAX> tp(taps);
-    const int w = xcd_remap(blockIdx.x, b.total_tiles);
+    // phases alternate the walk over each XCD's share of the list (the setup
+    // goes forward, phase 0 backward), so a phase starts on the tiles whose
+    // data the previous launch left in that XCD's L2 and in the MALL
+    const int w = xcd_remap(blockIdx.x, b.total_tiles, (ph & 1) == 0);
     const PairDesc &P = b.pairs[b.tile_pair[w]];
-    const HsImage im = phase_image(P, ph);
+    const HsImage im = phase_image(P, ph, b.phase0);
     hs_tile<NT, TW, TH, KMAX, NTAP, false, DB, FDIV>(im, w - P.tile_base, kk, tp,
                                                       uvm + Win::PAD);
-}
-
-template <int NT, int TW, int TH>
-__global__ __launch_bounds__(NT) void hs_multi_copy_kernel(BatchArgs b) {
-    copy_tile<NT, TW, TH>(b, xcd_remap(blockIdx.x, b.total_tiles));
 }
 
 // ---------------------------------------------------------------------------
 struct BVariant {
     int nt, tw, th, kmax, wpe;
     void (*kernel[2])(BatchArgs, SetupTaps, Taps);  // persistent: 8 taps, any count
-    // stepped form (kernel[] null): setup, phase (8 taps, any count), copy
+    // stepped form (kernel[] null): setup, phase (8 taps, any count)
     void (*setup)(BatchArgs, SetupTaps);
     void (*phase[2])(BatchArgs, Taps, int, int);
-    void (*copy)(BatchArgs);
 };
 
 #define HB_V(NT, TW, TH, K, WPE)                                                          \
     {NT, TW, TH, K, WPE,                                                                  \
      {hs_batched_kernel<NT, TW, TH, K, 8, WPE>, hs_batched_kernel<NT, TW, TH, K, 0, WPE>}, \
-     nullptr, {nullptr, nullptr}, nullptr}
+     nullptr, {nullptr, nullptr}}
 #define HB_D(NT, TW, TH, K, WPE)                                                          \
     {NT, TW, TH, K, WPE,                                                                  \
      {hs_batched_kernel<NT, TW, TH, K, 8, WPE, true>,                                     \
       hs_batched_kernel<NT, TW, TH, K, 0, WPE, true>},                                    \
-     nullptr, {nullptr, nullptr}, nullptr}
+     nullptr, {nullptr, nullptr}}
 #define HB_F(NT, TW, TH, K, WPE)                                                          \
     {NT, TW, TH, K, WPE,                                                                  \
      {hs_batched_kernel<NT, TW, TH, K, 8, WPE, true, true>,                               \
       hs_batched_kernel<NT, TW, TH, K, 0, WPE, true, true>},                              \
-     nullptr, {nullptr, nullptr}, nullptr}
+     nullptr, {nullptr, nullptr}}
 // stepped form (DB: double-buffered window, FDIV: reciprocal quotient)
 #define HB_S(NT, TW, TH, K, DB, FDIV)                                                     \
     {NT, TW, TH, K, 0, {nullptr, nullptr}, hs_multi_setup_kernel<NT, TW, TH>,            \
      {hs_multi_phase_kernel<NT, TW, TH, K, 8, DB, FDIV>,                                  \
-      hs_multi_phase_kernel<NT, TW, TH, K, 0, DB, FDIV>},                                 \
-     hs_multi_copy_kernel<NT, TW, TH>}
+      hs_multi_phase_kernel<NT, TW, TH, K, 0, DB, FDIV>}}
 static const BVariant kBVariants[] = {
     HB_V(1024, 64, 32, 4, 8),   // 0: two work-groups per CU
     HB_V(1024, 64, 32, 8, 8),   // 1: two work-groups per CU
@@ -391,6 +363,8 @@ static int launch_batch(const BatchPlan &P, const SetupTaps &st, const Taps &av,
     a.total_tiles = P.total_tiles;
     a.iters = iters;
     a.k = P.k;
+    const int nph = (iters + P.k - 1) / P.k;
+    a.phase0 = nph & 1;
     const BVariant &V = kBVariants[P.variant];
     if (V.kernel[0]) {
         hipLaunchKernelGGL(V.kernel[av.nd == 8 ? 0 : 1], dim3(P.grid), dim3(V.nt), 0, s, a, st,
@@ -401,15 +375,10 @@ static int launch_batch(const BatchPlan &P, const SetupTaps &st, const Taps &av,
     const dim3 grid(P.total_tiles), block(V.nt);
     hipLaunchKernelGGL(V.setup, grid, block, 0, s, a, st);
     HMX_LAUNCH_CHECK("hs_multi_setup");
-    const int nph = (iters + P.k - 1) / P.k;
     for (int ph = 0; ph < nph; ++ph) {
         const int kk = std::min(P.k, iters - ph * P.k);
         hipLaunchKernelGGL(V.phase[av.nd == 8 ? 0 : 1], grid, block, 0, s, a, av, ph, kk);
         HMX_LAUNCH_CHECK("hs_multi_phase");
-    }
-    if (nph & 1) {
-        hipLaunchKernelGGL(V.copy, grid, block, 0, s, a);
-        HMX_LAUNCH_CHECK("hs_multi_copy");
     }
     return HMX_OK;
 }
@@ -492,9 +461,62 @@ extern "C" int hmx_hs_batched_variant_info(int variant, int *tile_w, int *tile_h
     return HMX_OK;
 }
 
+// Device tables of a new plan: the pair descriptors followed by the tile ->
+// pair table, and the barrier / status / claim words.  On failure the caller
+// frees what was allocated (free_plan).
+static int build_plan(BatchPlan &P, const std::vector<PairDesc> &desc, int ntaps, int iters) {
+    const int npairs = (int)desc.size();
+    const int tiles = P.total_tiles;
+    int rc = resident_grid(P.variant, ntaps, tiles, P.device, P.grid);
+    if (rc) return rc;
+    std::vector<int> tile_pair(tiles);
+    for (int i = 0; i < npairs; ++i)
+        std::fill(tile_pair.begin() + desc[i].tile_base,
+                  tile_pair.begin() + desc[i].tile_base + desc[i].ntiles, i);
+    HMX_HIP_TRY(hipMalloc(&P.d_pairs, sizeof(PairDesc) * npairs + sizeof(int) * tiles));
+    const int64_t phases = (iters + P.k - 1) / P.k;
+    P.sync_words = 64 + (phases + 1) * kXcds * kClaimStride;
+    HMX_HIP_TRY(hipMalloc(&P.d_sync, sizeof(unsigned) * P.sync_words));
+    HMX_HIP_TRY(hipMemcpy(P.d_pairs, desc.data(), sizeof(PairDesc) * npairs,
+                          hipMemcpyHostToDevice));
+    HMX_HIP_TRY(hipMemcpy(P.d_pairs + npairs, tile_pair.data(), sizeof(int) * tiles,
+                          hipMemcpyHostToDevice));
+    return HMX_OK;
+}
+
+// Captures the plan's launch sequence (fill kernel, then the persistent kernel
+// or the setup and phase launches) into a graph on a private stream.
+static int capture_plan(BatchPlan &P, const SetupTaps &st, const Taps &av, int npairs, int iters,
+                        long long spin_limit) {
+    hipStream_t cs;
+    HMX_HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(cs);
+        return fail(HMX_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
+    }
+    const int rc = launch_batch(P, st, av, npairs, iters, spin_limit, cs);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(cs, &g);
+    (void)hipStreamDestroy(cs);
+    if (rc || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc ? rc : fail(HMX_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
+    }
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(HMX_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+    }
+    P.graph = g;
+    P.exec = x;
+    return HMX_OK;
+}
+
 static int batched_impl(const hmx_pair *pairs, int npairs, const SetupTaps &st, const Taps &av,
                         int iters, int variant, int k, int flags, long long spin_limit,
-                        hipStream_t s) {
+                        hipStream_t s, hipEvent_t done = nullptr) {
     HMX_REQUIRE(pairs && npairs >= 1, "hmx_hs_batched: no pairs");
     HMX_REQUIRE(iters >= 0, "hmx_hs_batched: iters must be >= 0");
     HMX_REQUIRE(av.rx <= 1 && av.ry <= 1 && av.nd <= HS_MAX_TAPS,
@@ -573,30 +595,10 @@ static int batched_impl(const hmx_pair *pairs, int npairs, const SetupTaps &st, 
         P.variant = variant;
         P.k = k;
         P.total_tiles = tiles;
-        int rc = resident_grid(variant, av.nd, tiles, device, P.grid);
-        if (rc) return rc;
-        std::vector<int> tile_pair(tiles);
-        for (int i = 0; i < npairs; ++i)
-            std::fill(tile_pair.begin() + desc[i].tile_base,
-                      tile_pair.begin() + desc[i].tile_base + desc[i].ntiles, i);
-        HMX_HIP_TRY(hipMalloc(&P.d_pairs, sizeof(PairDesc) * npairs + sizeof(int) * tiles));
-        const int64_t phases = (iters + k - 1) / k;
-        P.sync_words = 64 + (phases + 1) * kXcds * kClaimStride;
-        HMX_HIP_TRY(hipMalloc(&P.d_sync, sizeof(unsigned) * P.sync_words));
-        HMX_HIP_TRY(hipMemcpy(P.d_pairs, desc.data(), sizeof(PairDesc) * npairs,
-                              hipMemcpyHostToDevice));
-        HMX_HIP_TRY(hipMemcpy(P.d_pairs + npairs, tile_pair.data(), sizeof(int) * tiles,
-                              hipMemcpyHostToDevice));
-        if (flags & HMX_BATCH_GRAPH) {
-            hipStream_t cs;
-            HMX_HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-            HMX_HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-            rc = launch_batch(P, st, av, npairs, iters, spin_limit, cs);
-            hipError_t e = hipStreamEndCapture(cs, &P.graph);
-            (void)hipStreamDestroy(cs);
-            if (rc) return rc;
-            if (e != hipSuccess) return fail(HMX_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
-            HMX_HIP_TRY(hipGraphInstantiate(&P.exec, P.graph, nullptr, nullptr, 0));
+        int rc = build_plan(P, desc, av.nd, iters);
+        if (rc) {
+            free_plan(P);  // everything allocated before the failure
+            return rc;
         }
         if (g_plans.size() >= kMaxPlans) {
             free_plan(g_plans.back());
@@ -605,6 +607,12 @@ static int batched_impl(const hmx_pair *pairs, int npairs, const SetupTaps &st, 
         g_plans.push_front(std::move(P));
         plan = &g_plans.front();
     }
+    // A plan first made by an eager call (the tuner, a sync run) is captured
+    // when a graph call first asks for it.
+    if ((flags & HMX_BATCH_GRAPH) && !plan->exec && iters > 0) {
+        int rc = capture_plan(*plan, st, av, npairs, iters, spin_limit);
+        if (rc) return rc;
+    }
     if (iters == 0) return HMX_OK;
     if ((flags & HMX_BATCH_GRAPH) && plan->exec) {
         HMX_HIP_TRY(hipGraphLaunch(plan->exec, s));
@@ -612,7 +620,11 @@ static int batched_impl(const hmx_pair *pairs, int npairs, const SetupTaps &st, 
         int rc = launch_batch(*plan, st, av, npairs, iters, spin_limit, s);
         if (rc) return rc;
     }
-    if (flags & HMX_BATCH_SYNC) {
+    if (done) HMX_HIP_TRY(hipEventRecord(done, s));
+    // The persistent form's grid barrier can give up (a grid that is not
+    // co-resident); its status is read back after every call, so a partial
+    // result never goes unreported.  The stepped form has no barrier.
+    if ((flags & HMX_BATCH_SYNC) || kBVariants[plan->variant].kernel[0]) {
         unsigned status = 0;
         HMX_HIP_TRY(hipStreamSynchronize(s));
         HMX_HIP_TRY(hipMemcpy(&status, plan->d_sync + 1, 4, hipMemcpyDeviceToHost));
@@ -675,10 +687,16 @@ extern "C" int hmx_hs_batched_tune(const hmx_pair *pairs, int npairs, int ndx, c
                 continue;
             }
             if (rc) break;
+            // e1 is recorded right after the launches, before the status
+            // read-back of the persistent form; a timed run whose barrier
+            // gave up is discarded like a warm-up that did
             HMX_HIP_TRY(hipEventRecord(e0, s));
-            rc = batched_impl(pairs, npairs, st, av, iters, vi, k, 0, kDefaultSpinLimit, s);
+            rc = batched_impl(pairs, npairs, st, av, iters, vi, k, 0, kDefaultSpinLimit, s, e1);
+            if (rc == HMX_ERR_TIMEOUT) {
+                rc = HMX_OK;
+                continue;
+            }
             if (rc) break;
-            HMX_HIP_TRY(hipEventRecord(e1, s));
             HMX_HIP_TRY(hipEventSynchronize(e1));
             float ms = 0;
             HMX_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));

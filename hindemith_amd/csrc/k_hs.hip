@@ -39,6 +39,7 @@ struct HsArgs {
     const float *alpha2_b;  // per-pair alpha^2 (device), or null: use alpha2
     float alpha2;
     int H, W, k, tiles_x, ntiles;
+    int rev;                // walk each XCD's tiles backwards (see xcd_remap)
     int64_t bstride;        // elements between consecutive pairs (blockIdx.y)
     float *lt, *lub, *lvb;  // optional per-step linearisation output (see HsImage)
     int64_t lstride;
@@ -74,7 +75,7 @@ __global__ __launch_bounds__(NT, LOUT ? 8 : 1) void hs_tb_kernel(HsArgs a, Taps 
         im.lstride = a.lstride;
     }
     const HsTapTable<TW + 2 * KMAX> tp(taps);
-    hs_tile<NT, TW, TH, KMAX, NTAP, LOUT, DB, FDIV>(im, xcd_remap(blockIdx.x, a.ntiles), a.k,
+    hs_tile<NT, TW, TH, KMAX, NTAP, LOUT, DB, FDIV>(im, xcd_remap(blockIdx.x, a.ntiles, a.rev), a.k,
                                                     tp, uvm + Win::PAD, LOUT ? lin : nullptr);
 }
 
@@ -96,7 +97,8 @@ __global__ __launch_bounds__(NT) void hs_strip_kernel(HsArgs a, Taps taps) {
     im.W = a.W;
     im.tiles_x = a.tiles_x;
     const HsTapTable<TW + 2 * KMAX> tp(taps);
-    hs_tile_strip<NT, TW, TH, KMAX, S, NTAP>(im, xcd_remap(blockIdx.x, a.ntiles), a.k, tp, uv);
+    hs_tile_strip<NT, TW, TH, KMAX, S, NTAP>(im, xcd_remap(blockIdx.x, a.ntiles, a.rev), a.k, tp,
+                                             uv);
 }
 
 // ---------------------------------------------------------------------------
@@ -196,9 +198,10 @@ int launch_hs_tb(int variant, const float *Ix, const float *Iy, const float *It,
                  const float *u_in, const float *v_in, float *u_out, float *v_out,
                  hipStream_t s, int B = 1, const float *alpha2_b = nullptr,
                  float *lt = nullptr, float *lub = nullptr, float *lvb = nullptr,
-                 int64_t lstride = 0) {
+                 int64_t lstride = 0, bool rev = false) {
     const HsVariant &V = kVariants[variant];
     HsArgs a;
+    a.rev = rev ? 1 : 0;
     a.alpha2_b = alpha2_b;
     a.lt = lt;
     a.lub = lub;
@@ -245,28 +248,62 @@ int check_avg_taps(const Taps &t) {
     return HMX_OK;
 }
 
+// (u, v) -> (wu, wv): both planes of n pixels in one pass.
+__global__ __launch_bounds__(256) void hs_copy2_kernel(const float *__restrict__ u,
+                                                       const float *__restrict__ v,
+                                                       float *__restrict__ wu,
+                                                       float *__restrict__ wv, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+        wu[i] = u[i];
+        wv[i] = v[i];
+    }
+}
+
+int launch_hs_copy2(const float *u, const float *v, float *wu, float *wv, int64_t n,
+                    hipStream_t s) {
+    if (n <= 0) return HMX_OK;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(hs_copy2_kernel, dim3((unsigned)blocks), dim3(256), 0, s, u, v, wu, wv,
+                       n);
+    HMX_LAUNCH_CHECK("hs_copy2");
+    return HMX_OK;
+}
+
+// Launches of a solve of iters iterations at k per launch.
+int hs_launch_count(int iters, int k) { return iters <= 0 ? 0 : (iters + k - 1) / k; }
+
 // iters Jacobi steps, ping-ponging between (u,v) and (wu,wv); the result ends
-// in (u,v).
+// in (u,v).  Launches alternate direction, so with an odd launch count the
+// start state is first copied into (wu,wv) (one pass over both planes) and
+// the first launch reads it from there; start_in_work: the caller has already
+// put the start state there (hmx_hs_solve's setup does it in the same pass).
 int hs_jacobi_run(const float *Ix, const float *Iy, const float *It, const float *den,
                   float alpha2, int H, int W, const Taps &taps, int iters, float *u, float *v,
                   float *wu, float *wv, int variant, int k, hipStream_t s, int B,
-                  const float *alpha2_b) {
+                  const float *alpha2_b, bool start_in_work) {
     if (iters <= 0) return HMX_OK;
+    const bool odd = hs_launch_count(iters, k) & 1;
     float *src_u = u, *src_v = v, *dst_u = wu, *dst_v = wv;
+    if (odd) {
+        if (!start_in_work) {
+            int rc = launch_hs_copy2(u, v, wu, wv, (int64_t)H * W * B, s);
+            if (rc) return rc;
+        }
+        std::swap(src_u, dst_u);
+        std::swap(src_v, dst_v);
+    }
     int done = 0;
+    bool rev = false;  // alternate launches walk the tiles backwards (xcd_remap)
     while (done < iters) {
         const int kk = (iters - done) < k ? (iters - done) : k;
         int rc = launch_hs_tb(variant, Ix, Iy, It, den, alpha2, H, W, taps, kk, src_u, src_v,
-                              dst_u, dst_v, s, B, alpha2_b);
+                              dst_u, dst_v, s, B, alpha2_b, nullptr, nullptr, nullptr, 0, rev);
+        rev = !rev;
         if (rc) return rc;
         std::swap(src_u, dst_u);
         std::swap(src_v, dst_v);
         done += kk;
-    }
-    if (src_u != u) {
-        const size_t bytes = sizeof(float) * (size_t)H * W * B;
-        HMX_HIP_TRY(hipMemcpyAsync(u, src_u, bytes, hipMemcpyDeviceToDevice, s));
-        HMX_HIP_TRY(hipMemcpyAsync(v, src_v, bytes, hipMemcpyDeviceToDevice, s));
     }
     return HMX_OK;
 }
@@ -275,7 +312,7 @@ int hs_jacobi_run(const float *Ix, const float *Iy, const float *It, const float
                   float alpha2, int H, int W, const Taps &taps, int iters, float *u, float *v,
                   float *wu, float *wv, int variant, int k, hipStream_t s) {
     return hs_jacobi_run(Ix, Iy, It, den, alpha2, H, W, taps, iters, u, v, wu, wv, variant, k,
-                         s, 1, nullptr);
+                         s, 1, nullptr, false);
 }
 
 void hs_choice(int H, int W, int &variant, int &k) {
@@ -437,7 +474,7 @@ extern "C" int hmx_hs_tune(const float *Ix, const float *Iy, const float *It, fl
 namespace hmx {
 int launch_hs_setup(const float *I1, const float *I2, int H, int W, const Taps &dx,
                     const Taps &dy, const Taps &dt, float *Ix, float *Iy, float *It,
-                    hipStream_t s);
+                    hipStream_t s, const float *u, const float *v, float *wu, float *wv);
 }
 
 extern "C" int hmx_hs_solve(const float *I1, const float *I2, int H, int W, float alpha2,
@@ -459,10 +496,14 @@ extern "C" int hmx_hs_solve(const float *I1, const float *I2, int H, int W, floa
     float *Ix = workspace, *Iy = workspace + n, *It = workspace + 2 * n;
     float *wu = workspace + 3 * n, *wv = workspace + 4 * n;
     hipStream_t s = as_stream(stream);
-    rc = launch_hs_setup(I1, I2, H, W, dx, dy, dt, Ix, Iy, It, s);
-    if (rc) return rc;
     int variant, k;
     hs_choice(H, W, variant, k);
+    // an odd launch count starts from the work planes: the setup pass copies
+    // (u, v) there as it goes
+    const bool odd = hs_launch_count(iters, k) & 1;
+    rc = launch_hs_setup(I1, I2, H, W, dx, dy, dt, Ix, Iy, It, s, odd ? u : nullptr,
+                         odd ? v : nullptr, wu, wv);
+    if (rc) return rc;
     return hs_jacobi_run(Ix, Iy, It, nullptr, alpha2, H, W, av, iters, u, v, wu, wv, variant, k,
-                         s);
+                         s, 1, nullptr, odd);
 }

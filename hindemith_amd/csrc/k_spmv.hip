@@ -9,6 +9,7 @@
 // rows are binned by length and each bin gets its own L.
 #include "hmx_common.h"
 
+#include <algorithm>
 #include <vector>
 
 #pragma clang fp contract(off)
@@ -141,9 +142,149 @@ __global__ __launch_bounds__(RB_NT) void csr_rowblock_kernel(const int *__restri
     if (row < nrows && lane == 0) y[r0 + row] = acc;
 }
 
+// ---------------------------------------------------------------------------
+// Column-sliced CSR for rows whose columns are spread over the whole of x
+// (the power-law matrix: uniformly random columns, x = 16.8 MB, larger than
+// one XCD's 4 MB L2).  The columns are cut into SL_N = 8 equal ranges, one
+// per XCD, and each range's entries form their own CSR over all rows (built
+// once per sparsity pattern on the host; the values are permuted into that
+// order once per matrix on the device).  Work-group g serves slice g % 8,
+// and work-groups are dealt to the XCDs in that order, so every gather of x
+// an XCD makes falls in its own 2.1 MB slice, which stays in its L2 while the
+// matrix streams past with non-temporal loads.  Each work-group takes SL_RB
+// consecutive rows of its slice and writes one partial sum per row; a second
+// pass adds a row's 8 partials in slice order.  Within a slice a row's
+// products are added in entry order, so results are deterministic and within
+// the multi-lane bound of the fp64 product.
+constexpr int SL_N = 8, SL_RB = 256, SL_CAP = 2048;
+
+__global__ __launch_bounds__(SL_RB) void csr_slice_kernel(int M, const int *__restrict__ sptr,
+                                                          const int *__restrict__ sidx,
+                                                          const float *__restrict__ sdata,
+                                                          const float *__restrict__ x,
+                                                          float *__restrict__ partial) {
+    __shared__ float prod[SL_CAP];
+    const int s = blockIdx.x % SL_N;
+    const int r0 = (blockIdx.x / SL_N) * SL_RB;
+    if (r0 >= M) return;
+    const int r1 = min(r0 + SL_RB, M);
+    const int *ptr = sptr + (int64_t)s * (M + 1);
+    const int tid = threadIdx.x, row = r0 + tid;
+    const int beg = ptr[r0], end = ptr[r1];
+    int a = 0, e = 0;
+    if (row < r1) {
+        a = __builtin_nontemporal_load(ptr + row);
+        e = __builtin_nontemporal_load(ptr + row + 1);
+    }
+    float acc = 0.0f;
+    for (int cb = beg; cb < end; cb += SL_CAP) {
+        const int m = min(SL_CAP, end - cb);
+        for (int k = tid; k < m; k += SL_RB) {
+            const float d = __builtin_nontemporal_load(sdata + cb + k);
+            const int c = __builtin_nontemporal_load(sidx + cb + k);
+            prod[k] = d * x[c];
+        }
+        __syncthreads();
+        const int lo = max(a, cb) - cb, hi = min(e, cb + m) - cb;
+        for (int k = lo; k < hi; ++k) acc += prod[k];
+        __syncthreads();
+    }
+    if (row < r1) __builtin_nontemporal_store(acc, partial + (int64_t)s * M + row);
+}
+
+__global__ __launch_bounds__(256) void csr_slice_sum_kernel(int M, const float *__restrict__ partial,
+                                                            float *__restrict__ y) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= M) return;
+    float acc = __builtin_nontemporal_load(partial + i);
+#pragma unroll
+    for (int s = 1; s < SL_N; ++s) acc += __builtin_nontemporal_load(partial + (int64_t)s * M + i);
+    y[i] = acc;
+}
+
+__global__ __launch_bounds__(256) void csr_slice_values_kernel(int64_t nnz, const int *__restrict__ perm,
+                                                               const float *__restrict__ data,
+                                                               float *__restrict__ sdata) {
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < nnz; k += stride)
+        sdata[k] = data[perm[k]];
+}
+
 }  // namespace hmx
 
 using namespace hmx;
+
+extern "C" int hmx_csr_slice_count(void) { return SL_N; }
+
+extern "C" int hmx_csr_slice_build(int M, int N, const int *indptr_host, const int *indices_host,
+                                   int *sptr_host, int *sidx_host, int *perm_host) {
+    HMX_REQUIRE(M >= 0 && N >= 1 && indptr_host && sptr_host, "hmx_csr_slice_build: bad args");
+    const int64_t nnz = indptr_host[M];
+    HMX_REQUIRE(nnz == 0 || (indices_host && sidx_host && perm_host),
+                "hmx_csr_slice_build: null entry arrays");
+    const int sw = (N + SL_N - 1) / SL_N;
+    const int64_t P = (int64_t)M + 1;
+    // entries per (slice, row), then slice-major prefix sums
+    std::vector<int> cnt((size_t)SL_N * M, 0);
+    for (int i = 0; i < M; ++i) {
+        HMX_REQUIRE(indptr_host[i + 1] >= indptr_host[i], "hmx_csr_slice_build: indptr decreases");
+        for (int j = indptr_host[i]; j < indptr_host[i + 1]; ++j) {
+            const int c = indices_host[j];
+            HMX_REQUIRE(c >= 0 && c < N, "hmx_csr_slice_build: column index out of range");
+            ++cnt[(size_t)(c / sw) * M + i];
+        }
+    }
+    int64_t pos = 0;
+    for (int sl = 0; sl < SL_N; ++sl) {
+        int *ptr = sptr_host + sl * P;
+        for (int i = 0; i < M; ++i) {
+            ptr[i] = (int)pos;
+            pos += cnt[(size_t)sl * M + i];
+        }
+        ptr[M] = (int)pos;
+    }
+    // stable fill: a row's entries keep their order inside each slice
+    std::vector<int> cur((size_t)SL_N * M);
+    for (int sl = 0; sl < SL_N; ++sl)
+        std::copy(sptr_host + sl * P, sptr_host + sl * P + M, cur.begin() + (size_t)sl * M);
+    for (int i = 0; i < M; ++i)
+        for (int j = indptr_host[i]; j < indptr_host[i + 1]; ++j) {
+            const int c = indices_host[j];
+            const int k = cur[(size_t)(c / sw) * M + i]++;
+            sidx_host[k] = c;
+            perm_host[k] = j;
+        }
+    return HMX_OK;
+}
+
+extern "C" int hmx_csr_slice_values(int64_t nnz, const int *perm, const float *data,
+                                    float *sdata, void *stream) {
+    HMX_REQUIRE(nnz >= 0 && (nnz == 0 || (perm && data && sdata)),
+                "hmx_csr_slice_values: bad args");
+    if (nnz == 0) return HMX_OK;
+    const int64_t blocks = std::min<int64_t>((nnz + 255) / 256, 16384);
+    hipLaunchKernelGGL(csr_slice_values_kernel, dim3((unsigned)blocks), dim3(256), 0,
+                       as_stream(stream), nnz, perm, data, sdata);
+    HMX_LAUNCH_CHECK("csr_slice_values");
+    return HMX_OK;
+}
+
+extern "C" int hmx_csr_spmv_sliced(int M, const int *sptr, const int *sidx, const float *sdata,
+                                   const float *x, float *partial, float *y, void *stream) {
+    HMX_REQUIRE(M >= 0 && sptr && x && partial && y, "hmx_csr_spmv_sliced: null argument");
+    HMX_REQUIRE((const void *)x != (const void *)y, "hmx_csr_spmv_sliced: x and y must differ");
+    if (M == 0) return HMX_OK;
+    hipStream_t s = as_stream(stream);
+    const int64_t blocks = (int64_t)SL_N * ((M + SL_RB - 1) / SL_RB);
+    HMX_REQUIRE(blocks < ((int64_t)1 << 31), "hmx_csr_spmv_sliced: too many rows");
+    hipLaunchKernelGGL(csr_slice_kernel, dim3((unsigned)blocks), dim3(SL_RB), 0, s, M, sptr, sidx,
+                       sdata, x, partial);
+    HMX_LAUNCH_CHECK("csr_slice");
+    hipLaunchKernelGGL(csr_slice_sum_kernel, dim3((M + 255) / 256), dim3(256), 0, s, M, partial,
+                       y);
+    HMX_LAUNCH_CHECK("csr_slice_sum");
+    return HMX_OK;
+}
 
 extern "C" int hmx_csr_spmv(int M, int N, int64_t nnz, const int *indptr, const int *indices,
                             const float *data, const float *x, float *y, int lanes_per_row,

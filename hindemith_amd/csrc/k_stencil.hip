@@ -239,17 +239,25 @@ __device__ __forceinline__ float tap_apply4(const float *__restrict__ x, int H, 
     return acc;
 }
 
+// COPY: the same pass also copies the solve's start state (u, v) into the
+// work planes (wu, wv), where the first Jacobi launch reads it when the
+// launch count is odd.
+template <bool COPY>
 __global__ __launch_bounds__(256) void hs_setup4_kernel(const float *__restrict__ I1,
                                                         const float *__restrict__ I2, int H,
                                                         int W, Taps dx, Taps dy, Taps dt,
                                                         float *__restrict__ Ix,
                                                         float *__restrict__ Iy,
-                                                        float *__restrict__ It) {
+                                                        float *__restrict__ It,
+                                                        const float *__restrict__ u,
+                                                        const float *__restrict__ v,
+                                                        float *__restrict__ wu,
+                                                        float *__restrict__ wv) {
     const int c = blockIdx.x * 256 + threadIdx.x;
     if (c >= W) return;
 #pragma unroll
-    for (int v = 0; v < SETUP_V; ++v) {
-        const int r = blockIdx.y * SETUP_V + v;
+    for (int q = 0; q < SETUP_V; ++q) {
+        const int r = blockIdx.y * SETUP_V + q;
         if (r >= H) break;
         const int64_t p = (int64_t)r * W + c;
         const float a1 = tap_apply4(I1, H, W, r, c, dx);
@@ -261,20 +269,112 @@ __global__ __launch_bounds__(256) void hs_setup4_kernel(const float *__restrict_
         Ix[p] = a1 + a2;
         Iy[p] = b1 + b2;
         It[p] = t2 - t1;
+        if (COPY) {
+            wu[p] = u[p];
+            wv[p] = v[p];
+        }
     }
 }
 
+// The canonical cube's stencils all read the 2 x 2 block (c, r), (c+1, r),
+// (c, r+1), (c+1, r+1) (offsets in {0, 1}^2).  For that case a thread takes
+// one column and CUBE_V rows and reads the (CUBE_V + 1)-row, 2-column
+// neighbourhood of each frame once into registers (row r+1 of one output row
+// is row r of the next); each tap is then a register select on its
+// work-group-uniform offset, accumulated with fma in tap order from 0 as in
+// Stencil2DMVM.cpp:37-47.  Reads per pixel fall from 24 (L1-served) to
+// 4 (CUBE_V + 1) / CUBE_V.
+constexpr int CUBE_V = 8;
+
+bool taps_in_cube(const Taps &t) {
+    if (t.nd != 4) return false;
+    for (int d = 0; d < 4; ++d)
+        if (t.ox[d] < 0 || t.ox[d] > 1 || t.oy[d] < 0 || t.oy[d] > 1) return false;
+    return true;
+}
+
+// x00 = X(c, r), x10 = X(c+1, r), x01 = X(c, r+1), x11 = X(c+1, r+1), clamped
+__device__ __forceinline__ float cube_apply(const Taps &t, float x00, float x10, float x01,
+                                            float x11) {
+    float acc = 0.0f;
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+        const float lo = t.ox[d] ? x10 : x00, hi = t.ox[d] ? x11 : x01;
+        acc = fmaf(t.w[d], t.oy[d] ? hi : lo, acc);
+    }
+    return acc;
+}
+
+template <bool COPY>
+__global__ __launch_bounds__(256) void hs_setup_cube_kernel(
+    const float *__restrict__ I1, const float *__restrict__ I2, int H, int W, Taps dx, Taps dy,
+    Taps dt, float *__restrict__ Ix, float *__restrict__ Iy, float *__restrict__ It,
+    const float *__restrict__ u, const float *__restrict__ v, float *__restrict__ wu,
+    float *__restrict__ wv) {
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= W) return;
+    const int c1 = min(c + 1, W - 1);
+    const int r0 = blockIdx.y * CUBE_V;
+    float a0[CUBE_V + 1], a1[CUBE_V + 1], b0[CUBE_V + 1], b1[CUBE_V + 1];
+#pragma unroll
+    for (int j = 0; j <= CUBE_V; ++j) {
+        const int64_t row = (int64_t)min(r0 + j, H - 1) * W;
+        a0[j] = I1[row + c];
+        a1[j] = I1[row + c1];
+        b0[j] = I2[row + c];
+        b1[j] = I2[row + c1];
+    }
+#pragma unroll
+    for (int q = 0; q < CUBE_V; ++q) {
+        const int r = r0 + q;
+        if (r >= H) break;
+        const int64_t p = (int64_t)r * W + c;
+        const float e1 = cube_apply(dx, a0[q], a1[q], a0[q + 1], a1[q + 1]);
+        const float e2 = cube_apply(dx, b0[q], b1[q], b0[q + 1], b1[q + 1]);
+        const float f1 = cube_apply(dy, a0[q], a1[q], a0[q + 1], a1[q + 1]);
+        const float f2 = cube_apply(dy, b0[q], b1[q], b0[q + 1], b1[q + 1]);
+        const float g2 = cube_apply(dt, b0[q], b1[q], b0[q + 1], b1[q + 1]);
+        const float g1 = cube_apply(dt, a0[q], a1[q], a0[q + 1], a1[q + 1]);
+        Ix[p] = e1 + e2;
+        Iy[p] = f1 + f2;
+        It[p] = g2 - g1;
+        if (COPY) {
+            wu[p] = u[p];
+            wv[p] = v[p];
+        }
+    }
+}
+
+int launch_hs_copy2(const float *u, const float *v, float *wu, float *wv, int64_t n,
+                    hipStream_t s);
+
+// u non-null: also copy (u, v) into (wu, wv) (see hs_setup4_kernel).
 int launch_hs_setup(const float *I1, const float *I2, int H, int W, const Taps &dx,
                     const Taps &dy, const Taps &dt, float *Ix, float *Iy, float *It,
-                    hipStream_t s) {
-    if (dx.nd == 4 && dy.nd == 4 && dt.nd == 4 && (H + SETUP_V - 1) / SETUP_V <= 65535) {
+                    hipStream_t s, const float *u, const float *v, float *wu, float *wv) {
+    if (taps_in_cube(dx) && taps_in_cube(dy) && taps_in_cube(dt) &&
+        (H + CUBE_V - 1) / CUBE_V <= 65535) {
+        dim3 grid((W + 255) / 256, (H + CUBE_V - 1) / CUBE_V);
+        if (u)
+            hipLaunchKernelGGL(hs_setup_cube_kernel<true>, grid, dim3(256), 0, s, I1, I2, H, W,
+                               dx, dy, dt, Ix, Iy, It, u, v, wu, wv);
+        else
+            hipLaunchKernelGGL(hs_setup_cube_kernel<false>, grid, dim3(256), 0, s, I1, I2, H, W,
+                               dx, dy, dt, Ix, Iy, It, nullptr, nullptr, nullptr, nullptr);
+    } else if (dx.nd == 4 && dy.nd == 4 && dt.nd == 4 && (H + SETUP_V - 1) / SETUP_V <= 65535) {
         dim3 grid((W + 255) / 256, (H + SETUP_V - 1) / SETUP_V);
-        hipLaunchKernelGGL(hs_setup4_kernel, grid, dim3(256), 0, s, I1, I2, H, W, dx, dy, dt,
-                           Ix, Iy, It);
+        if (u)
+            hipLaunchKernelGGL(hs_setup4_kernel<true>, grid, dim3(256), 0, s, I1, I2, H, W, dx,
+                               dy, dt, Ix, Iy, It, u, v, wu, wv);
+        else
+            hipLaunchKernelGGL(hs_setup4_kernel<false>, grid, dim3(256), 0, s, I1, I2, H, W, dx,
+                               dy, dt, Ix, Iy, It, nullptr, nullptr, nullptr, nullptr);
     } else {
         dim3 grid((W + 255) / 256, H < 65535 ? H : 65535);
         hipLaunchKernelGGL(hs_setup_kernel, grid, dim3(256), 0, s, I1, I2, H, W, dx, dy, dt, Ix,
                            Iy, It);
+        HMX_LAUNCH_CHECK("hs_setup");
+        if (u) return launch_hs_copy2(u, v, wu, wv, (int64_t)H * W, s);
     }
     HMX_LAUNCH_CHECK("hs_setup");
     return HMX_OK;
@@ -349,5 +449,6 @@ extern "C" int hmx_hs_setup(const float *I1, const float *I2, int H, int W, int 
     if (!rc) rc = make_taps(dy, ndy, wy, oxy, oyy);
     if (!rc) rc = make_taps(dt, ndt, wt, oxt, oyt);
     if (rc) return rc;
-    return launch_hs_setup(I1, I2, H, W, dx, dy, dt, Ix, Iy, It, as_stream(stream));
+    return launch_hs_setup(I1, I2, H, W, dx, dy, dt, Ix, Iy, It, as_stream(stream), nullptr,
+                           nullptr, nullptr, nullptr);
 }

@@ -18,6 +18,7 @@ from ._lib import HindemithError, check, lib
 
 __all__ = [
     "taps_of", "stencil2d", "dia2d_spmv", "csr_spmv", "CSRBins", "csr_spmv_binned",
+    "CSRRowBlocks", "csr_spmv_rowblocks", "CSRSlices", "csr_spmv_sliced",
     "sum2d", "hs_setup", "hs_jacobi", "hs_solve", "hs_tune", "hs_variants", "stream_ptr",
     "hs_precondition", "hs_matrix_powers", "warp_img2d", "dia_spmv", "brox_set_matrix",
     "brox_set_redblack_matrix", "brox_precondition",
@@ -145,6 +146,51 @@ def csr_spmv_rowblocks(indptr, indices, data, x, rb, out=None):
     check(lib().hmx_csr_spmv_rowblocks(rb.M, _ptr(indptr), _ptr(indices), _ptr(data), _ptr(x),
                                        _ptr(y), _ptr(rb.blocks), rb.nblocks, stream_ptr()),
           "csr_spmv_rowblocks")
+    return y
+
+
+class CSRSlices:
+    """Column-sliced form of a CSR matrix (one column range per XCD; built once
+    per sparsity pattern on the host) for hmx_csr_spmv_sliced.  ``values(data)``
+    permutes a device value array of the original matrix into slice order."""
+
+    def __init__(self, indptr_host, indices_host, n_cols, device):
+        ip = np.ascontiguousarray(indptr_host, np.int32)
+        ix = np.ascontiguousarray(indices_host, np.int32)
+        M = len(ip) - 1
+        nnz = int(ip[-1])
+        L = lib()
+        S = L.hmx_csr_slice_count()
+        sptr = np.empty(S * (M + 1), np.int32)
+        sidx = np.empty(max(nnz, 1), np.int32)
+        perm = np.empty(max(nnz, 1), np.int32)
+        check(L.hmx_csr_slice_build(M, int(n_cols), ip.ctypes.data_as(ctypes.c_void_p),
+                                    ix.ctypes.data_as(ctypes.c_void_p),
+                                    sptr.ctypes.data_as(ctypes.c_void_p),
+                                    sidx.ctypes.data_as(ctypes.c_void_p),
+                                    perm.ctypes.data_as(ctypes.c_void_p)), "csr_slice_build")
+        self.M, self.nnz, self.slices = M, nnz, S
+        self.sptr = torch.from_numpy(sptr).to(device)
+        self.sidx = torch.from_numpy(sidx).to(device)
+        self.perm = torch.from_numpy(perm).to(device)
+        self.partial = torch.empty(max(S * M, 1), device=device, dtype=torch.float32)
+
+    def values(self, data, out=None):
+        _dev(data, "data")
+        sd = torch.empty(max(self.nnz, 1), device=data.device, dtype=torch.float32) \
+            if out is None else out
+        check(lib().hmx_csr_slice_values(self.nnz, _ptr(self.perm), _ptr(data), _ptr(sd),
+                                         stream_ptr()), "csr_slice_values")
+        return sd
+
+
+def csr_spmv_sliced(sl, sdata, x, out=None):
+    """y = A * x from the column-sliced form (sl: CSRSlices of A, sdata:
+    sl.values(data))."""
+    _dev(x, "x")
+    y = torch.empty(sl.M, device=x.device, dtype=torch.float32) if out is None else out
+    check(lib().hmx_csr_spmv_sliced(sl.M, _ptr(sl.sptr), _ptr(sl.sidx), _ptr(sdata), _ptr(x),
+                                    _ptr(sl.partial), _ptr(y), stream_ptr()), "csr_spmv_sliced")
     return y
 
 

@@ -140,6 +140,28 @@ int hmx_csr_spmv_rowblocks(int M, const int *indptr, const int *indices, const f
                            const float *x, float *y, const int *blocks, int nblocks,
                            void *stream);
 
+/* Column-sliced CSR SpMV for irregular rows with columns spread over all of x
+ * (same product as SpmvCSR, SpmvCSR.cpp:30-47; its sums regrouped).  The
+ * columns are cut into hmx_csr_slice_count() = 8 equal ranges, one per XCD of
+ * the MI355X, and each range's entries form their own CSR over all M rows,
+ * so the gathers of x one XCD makes stay within a range that fits its L2.
+ * hmx_csr_slice_build (host, once per sparsity pattern): sptr_host
+ * (8 * (M + 1) ints) receives each slice's row offsets into the sliced entry
+ * arrays, sidx_host (nnz ints) the column indices in slice order, perm_host
+ * (nnz ints) the original position of each sliced entry; a row keeps its
+ * entry order inside each slice.  hmx_csr_slice_values (device, once per
+ * matrix values): sdata[k] = data[perm[k]].  hmx_csr_spmv_sliced: y = A * x;
+ * partial is device scratch of 8 * M floats; each row's products are added
+ * in entry order within a slice and the 8 slice sums in slice order
+ * (deterministic; error within the multi-lane bound). */
+int hmx_csr_slice_count(void);
+int hmx_csr_slice_build(int M, int N, const int *indptr_host, const int *indices_host,
+                        int *sptr_host, int *sidx_host, int *perm_host);
+int hmx_csr_slice_values(int64_t nnz, const int *perm, const float *data, float *sdata,
+                         void *stream);
+int hmx_csr_spmv_sliced(int M, const int *sptr, const int *sidx, const float *sdata,
+                        const float *x, float *partial, float *y, void *stream);
+
 /* a7. out[0] = sum of x[0..n) (sum2d, Array2DSum.cpp:12-250).  Deterministic
  *     two-pass reduction with fp64 accumulation.  partials: device scratch of
  *     HMX_SUM_PARTIALS doubles.  out: device float. */
@@ -289,17 +311,20 @@ struct hmx_pair {
 /* Setup + iters Jacobi iterations of every pair over one list of all pairs'
  * tiles, in phases of k iterations.  Two forms:
  *   stepped (variants 8-11, the default): one launch per phase over every
- *     tile (plus a setup launch and, after an odd number of phases, a copy);
+ *     tile, plus a setup launch (which, for an odd number of phases, also
+ *     copies (u, v) into the work planes so the last phase writes (u, v));
  *   persistent (variants 0-7): ONE launch whose grid, no larger than the GPU
  *     holds at once, walks the list with a grid-wide barrier between phases;
  *     each of the 8 XCDs owns a contiguous share of the list and its
  *     work-groups claim tiles from it through a per-phase counter.  Barrier
  *     waits give up after spin_limit polls (<= 0: 2^24, about 3 s) and then
- *     every work-group exits; with HMX_BATCH_SYNC the call returns
- *     HMX_ERR_TIMEOUT in that case.
+ *     every work-group exits.  A persistent call always synchronises its
+ *     stream and returns HMX_ERR_TIMEOUT in that case (so a partial result
+ *     is never silent); HMX_BATCH_SYNC makes a stepped call synchronise too.
  * variant = -1 uses the autotuned (or default) variant and k.
  * Launch state (descriptors, barrier words, graph) is cached per signature;
- * calls that overlap on one stream must not share pairs. */
+ * a graph call on a signature first run eagerly captures the graph then.
+ * Calls that overlap on one stream must not share pairs. */
 int hmx_hs_batched(const struct hmx_pair *pairs, int npairs,
                    int ndx, const float *wx, const int *oxx, const int *oyx,
                    int ndy, const float *wy, const int *oxy, const int *oyy,

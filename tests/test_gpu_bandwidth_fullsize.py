@@ -10,9 +10,10 @@ from seed 1).
   row is the reference's sequential fma chain, bit-exact; the lane heuristic's
   choice within the multi-lane bound of the fp64 product.
 * A Zipf-1.5 power-law CSR (4,194,304 rows, 1-512 entries per row, ~72.8 M
-  entries) through the row-block, row-binned and one-sub-wave-per-row launches:
-  within (len + 2) * eps32 * sum |a||x| of the fp64 product per row, and the
-  row-block result bit-identical on a second call.
+  entries) through the row-block, row-binned, one-sub-wave-per-row and
+  column-sliced launches: within (len + 2) * eps32 * sum |a||x| of the fp64
+  product per row, and the row-block and column-sliced results bit-identical
+  on a second call.
 """
 import numpy as np
 import pytest
@@ -98,7 +99,7 @@ def powerlaw():
     return ip, ix, d
 
 
-@pytest.mark.parametrize("form", ["rowblocks", "binned", "unbinned"])
+@pytest.mark.parametrize("form", ["rowblocks", "binned", "unbinned", "sliced"])
 def test_csr_powerlaw_2048(K, x_host, powerlaw, form):
     ip, ix, d = powerlaw
     nnz = int(ip[-1])
@@ -110,6 +111,13 @@ def test_csr_powerlaw_2048(K, x_host, powerlaw, form):
         y = _host(K.csr_spmv_rowblocks(tip, tix, td, tx, rb))
         y2 = _host(K.csr_spmv_rowblocks(tip, tix, td, tx, rb))
         assert y.tobytes() == y2.tobytes()
+    elif form == "sliced":
+        sl = K.CSRSlices(ip, ix, N, "cuda")
+        sd = sl.values(td)
+        y = _host(K.csr_spmv_sliced(sl, sd, tx))
+        y2 = _host(K.csr_spmv_sliced(sl, sd, tx))
+        assert y.tobytes() == y2.tobytes()
+        del sl, sd
     elif form == "binned":
         y = _host(K.csr_spmv_binned(tip, tix, td, tx, K.CSRBins(ip, "cuda")))
     else:

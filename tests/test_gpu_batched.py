@@ -93,6 +93,31 @@ def test_forced_barrier_timeout_drains_and_recovers():
     check_oracle(pairs, out, 16)
 
 
+def test_forced_barrier_timeout_reported_without_sync():
+    """A persistent call reports a barrier give-up even when the caller did not
+    ask for a synchronising call (the result would be partial)."""
+    from hindemith_amd._lib import HindemithError
+    pairs, B = make(12, seed=5, side=(200, 400), iters=16)
+    with pytest.raises(HindemithError, match="gave up"):
+        B.run(graph=False, sync=False, spin_limit=1, variant=1, k=8)
+    with pytest.raises(HindemithError, match="gave up"):
+        B.run(graph=True, sync=False, spin_limit=1, variant=1, k=8)
+    out = B.run(graph=False, sync=False, variant=1, k=8)
+    torch.cuda.synchronize()
+    check_oracle(pairs, out, 16)
+
+
+def test_graph_call_on_a_plan_made_eagerly():
+    """A signature first run eagerly (as the tuner does) is captured when a
+    graph call first asks for it; every replay equals the oracle."""
+    pairs, B = make(5, seed=8, iters=21)
+    B.run(graph=False, sync=True)
+    for _ in range(3):
+        out = B.run(graph=True)
+        torch.cuda.synchronize()
+        check_oracle(pairs, out, 21)
+
+
 def test_batched_equals_per_pair_plain_solves():
     pairs, B = make(8, seed=6, iters=25)
     batched = [(u.clone(), v.clone()) for u, v in B.run(graph=True, sync=True)]

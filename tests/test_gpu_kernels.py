@@ -152,6 +152,49 @@ def test_csr_rowblocks(K, case):
     assert np.all(np.abs(y1 - ref) <= (lens + 2) * np.finfo(np.float32).eps * absdot + 1e-30)
 
 
+@pytest.mark.parametrize("case", ["random", "powerlaw", "long_rows", "tiny", "empty_rows",
+                                  "narrow"])
+def test_csr_sliced(K, case):
+    """Column-sliced CSR SpMV: within the multi-lane bound of the fp64 product
+    (rows longer than the staging chunk, fewer columns than slices and empty
+    rows included) and bit-identical on a second call."""
+    rng = np.random.default_rng(13)
+    if case == "random":
+        M, N = 3001, 2500
+        indptr, indices, data = _random_csr(M, N, rng)
+    elif case == "powerlaw":
+        M = N = 20000
+        indptr, indices, data = S.powerlaw_csr(N, rng)
+    elif case == "long_rows":  # 5000 and 3000 entries in a few columns: > 2048 per slice
+        M, N = 40, 64
+        indptr, indices, data = _random_csr(M, N, rng, max_len=10)
+        lens = np.diff(indptr)
+        lens[3], lens[17] = 5000, 3000
+        indptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+        indices = rng.integers(0, N, int(indptr[-1])).astype(np.int32)
+        data = rng.normal(size=int(indptr[-1])).astype(np.float32)
+    elif case == "tiny":
+        M, N = 1, 1
+        indptr, indices, data = (np.array([0, 1], np.int32), np.array([0], np.int32),
+                                 np.array([2.5], np.float32))
+    elif case == "narrow":  # fewer columns than slices
+        M, N = 500, 3
+        indptr, indices, data = _random_csr(M, N, rng, max_len=6)
+    else:
+        M, N = 700, 900
+        indptr, indices, data = _random_csr(M, N, rng, empty_frac=0.8)
+    x = rng.uniform(-1, 1, N).astype(np.float32)
+    sl = K.CSRSlices(indptr, indices, N, "cuda")
+    sd = sl.values(dev(data))
+    y1 = host(K.csr_spmv_sliced(sl, sd, dev(x)))
+    y2 = host(K.csr_spmv_sliced(sl, sd, dev(x)))
+    assert y1.tobytes() == y2.tobytes()
+    ref = oracle.csr_spmv(indptr, indices, data, x, dtype=np.float64)
+    absdot = oracle.csr_spmv(indptr, indices, np.abs(data), np.abs(x), dtype=np.float64)
+    lens = np.diff(indptr)
+    assert np.all(np.abs(y1 - ref) <= (lens + 2) * np.finfo(np.float32).eps * absdot + 1e-30)
+
+
 @pytest.mark.parametrize("n", [0, 1, 1000, 2048 * 2048 + 17])
 def test_sum2d(K, n):
     rng = np.random.default_rng(n)

@@ -8,6 +8,9 @@
 // The grid never depends on timing, so the result is bit-reproducible.
 #include "hmx_common.h"
 
+#include <mutex>
+#include <vector>
+
 namespace hmx {
 
 constexpr int RED_NT = 256;
@@ -68,5 +71,45 @@ int launch_sum(const float *x, int64_t n, double *partials, float *out, double *
 extern "C" int hmx_sum2d(const float *x, int64_t n, double *partials, float *out, void *stream) {
     HMX_REQUIRE(partials && out, "hmx_sum2d: null partials/out");
     HMX_REQUIRE(n >= 0 && (n == 0 || x), "hmx_sum2d: bad input");
+    return hmx::launch_sum(x, n, partials, out, nullptr, hmx::as_stream(stream));
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free form of hmx_sum2d.  The pass-1 partials live in a
+// file-scope device array instead of a caller workspace; every GPU the
+// library runs on holds its own copy until the process ends.  In host code
+// the array's name is only a stand-in: the device address of this GPU's copy
+// comes from hipGetSymbolAddress, taken once per device on the first call
+// with that device current, and is the only form host code passes on.
+static __device__ __align__(256) double scratch[HMX_SUM_PARTIALS];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<double *> g_scratch_ptr;  // by device ordinal; null until set up
+}  // namespace
+
+static int sum_scratch(double **out) {
+    int dev = 0;
+    HMX_HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        HMX_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return hmx::fail(HMX_ERR_HIP, "hmx_sum2d_noalloc: no device address for scratch");
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return HMX_OK;
+}
+
+extern "C" int hmx_sum2d_noalloc(const float *x, int64_t n, float *out, void *stream) {
+    HMX_REQUIRE(out, "hmx_sum2d_noalloc: null out");
+    HMX_REQUIRE(n >= 0 && (n == 0 || x), "hmx_sum2d_noalloc: bad input");
+    // the partials are one per pass-1 work-group of the fixed grid, whatever n is
+    static_assert(sizeof(scratch) == sizeof(double) * HMX_SUM_PARTIALS, "scratch size");
+    double *partials = nullptr;
+    int rc = sum_scratch(&partials);
+    if (rc) return rc;
     return hmx::launch_sum(x, n, partials, out, nullptr, hmx::as_stream(stream));
 }

@@ -168,6 +168,18 @@ int hmx_csr_spmv_sliced(int M, const int *sptr, const int *sidx, const float *sd
 #define HMX_SUM_PARTIALS 1024
 int hmx_sum2d(const float *x, int64_t n, double *partials, float *out, void *stream);
 
+/* Allocation-free form of hmx_sum2d.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The pass-1
+ * partials live in a file-scope device array of HMX_SUM_PARTIALS doubles
+ * (8 KiB per GPU, held until the process ends); its device address is taken
+ * once per device, on the first call with that device current, so make that
+ * first call outside graph capture.  The partials grid is fixed, so every n
+ * fits the array.  All calls on one GPU share its array: calls that overlap
+ * on one GPU (other streams or threads) race.  Same result as hmx_sum2d, bit
+ * for bit.  out: device float. */
+int hmx_sum2d_noalloc(const float *x, int64_t n, float *out, void *stream);
+
 /* a4-a8. Fused elementwise program over n elements (Array2DOp,
  * Array2DScalarOp, ScalarArray2DOp, Array2DCopy, Array2DSqrt and stencil /
  * DIA taps of Stencil2DMVM / SpmvDIA2D).  The record runtime encodes its

@@ -8,10 +8,13 @@ borders.  One step = one full solve (derivative setup + 500 fused Jacobi
 iterations) with inputs already resident in HBM.
 
 Prints one JSON line: value = Mpixel*iter/s, plus
-  roofline: the fused Jacobi kernel's algorithmic bytes (28 B per pixel per
-            iteration: read u, v, Ix, Iy, It; write u, v) per launch over its
-            average launch time, timed with HIP events on the launch stream,
-            against the 8 TB/s HBM peak;
+  roofline: the fused Jacobi kernel against the resource that binds it, the
+            vector ALU: issued VALU lane-instructions per launch (rocprofv3
+            SQ_INSTS_VALU of the committed counter summary) over the mean
+            launch time, timed with HIP events on the launch stream, against
+            the non-packed fp32 issue peak; with the same launches against HBM
+            (28 B per pixel per iteration of algorithmic bytes, and the
+            measured FETCH/WRITE traffic) and against LDS;
   cpu_baseline: the oracle's C/OpenMP restatement on this host's cores, on a
             bounded sample of the same workload.
 """
@@ -28,6 +31,13 @@ sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_PX_ITER = 28
+# Non-packed fp32 VALU issue peak of MI355X: 256 CUs x 4 SIMDs x 16 lanes x
+# 2.4 GHz lane-instructions per second (a wave64 instruction occupies a SIMD
+# for 4 cycles).  Packed fp32 is disabled in these builds.
+VALU_LANE_PEAK = 256 * 4 * 16 * 2.4e9
+# LDS: 256 CUs x 128 B/clk x 2.4 GHz.
+LDS_PEAK_BPS = 256 * 128 * 2.4e9
+COUNTERS = os.path.join(HERE, "profiles", "hs_jacobi_1920x1080_counters.json")
 
 
 def parse():
@@ -425,18 +435,60 @@ def extra_adjoint(S):
             "backward_ms": round(b, 3), "backward_over_forward": round(b / f, 2)}
 
 
-def load_traffic(variant, k, H, W):
-    """HBM bytes per launch from the committed rocprofv3 PMC summary, if it
-    was measured for this exact configuration (variant, k, H, W)."""
-    path = os.path.join(HERE, "profiles", "hs_jacobi_1920x1080_pmc.json")
+def load_counters(variant, k, H, W, iters):
+    """Per-launch rocprofv3 counters of the fused Jacobi kernel from the
+    committed summary, if they were measured for this exact solve (variant, k,
+    H, W, iters): the mean over every launch of a solve."""
     try:
-        d = json.load(open(path))
+        d = json.load(open(COUNTERS))
     except (OSError, ValueError):
         return None
-    if d.get("H") != H or d.get("W") != W:
+    if (d.get("H"), d.get("W"), d.get("variant"), d.get("k"), d.get("iters")) != \
+            (H, W, variant, k, iters):
         return None
-    e = d.get("configs", {}).get(f"v{variant}k{k}")
-    return e.get("hbm_bytes_per_launch") if e else None
+    return d
+
+
+def hs_roofline(vinfo, variant, k, H, W, iters, launches, jac_ms):
+    """Roofline of the fused Jacobi kernel over one solve's launches.
+
+    The binding resource is the vector ALU: the blocked kernel moves about a
+    seventh of its algorithmic bytes through HBM, and its launches are spent
+    issuing VALU instructions (the step loop) and waiting on the load phase.
+    achieved = issued VALU instructions per launch (SQ_INSTS_VALU, counted per
+    wave) x 64 lanes / the mean launch time measured live with HIP events;
+    peak = the non-packed issue rate VALU_LANE_PEAK.  Also reported: the same
+    launches against HBM (algorithmic bytes, the figure the 28 B/px.iter
+    definition gives, and the measured FETCH/WRITE traffic) and against LDS."""
+    t = jac_ms * 1e-3 / launches
+    alg = BYTES_PER_PX_ITER * H * W * iters // launches
+    hbm = {"algorithmic_bytes_per_launch": alg,
+           "achieved_algorithmic_GBps": round(alg / t / 1e9, 1),
+           "frac_algorithmic": round(alg / t / 1e9 / HBM_PEAK_GBS, 4)}
+    base = {"kernel": vinfo["kernel"], "launches_per_step": launches,
+            "avg_launch_us": round(t * 1e6, 2)}
+    d = load_counters(variant, k, H, W, iters)
+    if d is None:  # counters not measured for this solve: the HBM roof only
+        return dict({"bound": "hbm", "achieved": hbm["achieved_algorithmic_GBps"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": hbm["frac_algorithmic"],
+                     "traffic": None, "hbm": hbm}, **base)
+    c = d["per_launch"]
+    traffic = int(c["FETCH_SIZE"] * 1024 * d["fetch_scale"] + c["WRITE_SIZE"] * 1024 *
+                  d["write_scale"])
+    hbm.update({"traffic_bytes_per_launch": traffic,
+                "achieved_measured_GBps": round(traffic / t / 1e9, 1),
+                "frac_measured": round(traffic / t / 1e9 / HBM_PEAK_GBS, 4)})
+    lanes = c["SQ_INSTS_VALU"] * 64
+    lds_bytes = (c["SQ_INSTS_LDS_LOAD_BANDWIDTH"] + c["SQ_INSTS_LDS_STORE_BANDWIDTH"]) * 64
+    lds = {"bytes_per_launch": int(lds_bytes), "achieved_TBps": round(lds_bytes / t / 1e12, 2),
+           "peak_TBps": round(LDS_PEAK_BPS / 1e12, 1),
+           "frac": round(lds_bytes / t / LDS_PEAK_BPS, 4)}
+    return dict({"bound": "valu", "achieved": round(lanes / t / 1e12, 2),
+                 "peak": round(VALU_LANE_PEAK / 1e12, 2), "unit": "Tlane-instr/s",
+                 "frac": round(lanes / t / VALU_LANE_PEAK, 4), "traffic": traffic,
+                 "valu_insts_per_launch": int(c["SQ_INSTS_VALU"]),
+                 "fp32_tflops": round(c["SQ_INSTS_VALU_FLOPS_FP32"] * 64 / t / 1e12, 2),
+                 "counters": os.path.relpath(COUNTERS, HERE), "hbm": hbm, "lds": lds}, **base)
 
 
 def main():
@@ -467,8 +519,8 @@ def main():
     dev = torch.device("cuda:0")
     I1 = torch.from_numpy(I1h).to(dev)
     I2 = torch.from_numpy(I2h).to(dev)
-    u = torch.zeros((H, W), device=dev)
-    v = torch.zeros((H, W), device=dev)
+    uv = torch.zeros((2, H, W), device=dev)  # u and v, cleared by one fill per step
+    u, v = uv[0], uv[1]
     ws = torch.empty((5, H, W), device=dev)
     Ix, Iy, It = ws[0], ws[1], ws[2]
     work = ws[3:5]
@@ -477,13 +529,12 @@ def main():
     if args.variant is not None:
         variant, k = args.variant, args.k or K.hs_variants()[args.variant]["kmax"]
     elif args.no_tune:
-        variant, k = None, None
+        variant, k = K.hs_config(H, W)
     else:
         variant, k = K.hs_tune(Ix, Iy, It, 1.0, taps["Avg"])
 
     def step(ev=None):
-        u.zero_()
-        v.zero_()
+        uv.zero_()
         K.hs_setup(I1, I2, taps["Dx"], taps["Dy"], taps["Dt"], out=ws[0:3])
         if ev is not None:
             ev[0].record()
@@ -535,26 +586,13 @@ def main():
                                                out=ws[0:3]), 5)
         jac_ms = max(jac_ms - setup_ms, 1e-6)
 
-    if variant is None:
-        vinfo = None
-        launches = None
-    else:
-        vinfo = K.hs_variants()[variant]
-        launches = -(-iters // k)
+    vinfo = K.hs_variants()[variant]
+    launches = -(-iters // k)
     ms_per_step = elapsed * 1e3 / args.steps
     value = H * W * iters / (ms_per_step * 1e-3) / 1e6
-    # algorithmic bytes of the whole solve over the Jacobi launches' time: the
-    # mean launch carries iters / launches iterations (a solve of 500 at k = 8
-    # is 62 launches of 8 and one of 4)
-    achieved = BYTES_PER_PX_ITER * H * W * iters / (jac_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": load_traffic(variant, k, H, W),
-                "kernel": vinfo["kernel"] if vinfo else "hs_tb_kernel",
-                "launches_per_step": launches,
-                "avg_launch_us": round(jac_ms * 1e3 / launches, 2) if launches else None,
-                "algorithmic_bytes_per_launch": (BYTES_PER_PX_ITER * H * W * iters // launches
-                                                 if launches else None)}
+    # the mean launch carries iters / launches iterations (a solve of 500 at
+    # k = 8 is 62 launches of 8 and one of 4)
+    roofline = hs_roofline(vinfo, variant, k, H, W, iters, launches, jac_ms)
 
     u_h = u.cpu().numpy()
     v_h = v.cpu().numpy()
@@ -570,7 +608,7 @@ def main():
         "config": {"workload": f"horn_schunck_jacobi_{W}x{H}_fp32_{iters}it", "height": H,
                    "width": W, "iters": iters, "alpha": 1.0,
                    "kernel_variant": variant, "iters_per_launch": k,
-                   "tile": [vinfo["tile_w"], vinfo["tile_h"]] if vinfo else None,
+                   "tile": [vinfo["tile_w"], vinfo["tile_h"]],
                    "hbm_roofline_fraction_of_metric": round(value / (HBM_PEAK_GBS * 1e9 /
                                                                      BYTES_PER_PX_ITER / 1e6), 4),
                    "endpoint_error_px": round(epe, 4),

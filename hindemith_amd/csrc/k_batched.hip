@@ -43,8 +43,11 @@ struct PairDesc {
 };
 
 struct BatchArgs {
-    const PairDesc *pairs;
-    const int *tile_pair;  // pair index of every tile of the list
+    // the descriptor of each tile's pair, one copy per tile of the list, so a
+    // work-group finds its pair with one load whose address it knows at once
+    // (a pair index followed by the pair's descriptor would be two dependent
+    // loads ahead of every tile)
+    const PairDesc *tiles;
     unsigned *claims;      // per (phase, XCD) tile-claim counters, kClaimStride apart
     unsigned *barrier;
     int *status;
@@ -128,7 +131,7 @@ __device__ __forceinline__ float setup_tap4(const float *__restrict__ x, int H, 
 // cube), taken by the unrolled form.
 template <int NT, int TW, int TH, bool TAP4 = false>
 __device__ __forceinline__ void setup_tile(const BatchArgs &b, const SetupTaps &st, int w) {
-    const PairDesc &P = b.pairs[b.tile_pair[w]];
+    const PairDesc &P = b.tiles[w];
     const int t = w - P.tile_base;
     const int tyi = t / P.tiles_x, txi = t - tyi * P.tiles_x;
     for (int i = threadIdx.x; i < TW * TH; i += NT) {
@@ -221,7 +224,7 @@ __global__ __launch_bounds__(NT, WPE) void hs_batched_kernel(BatchArgs b, SetupT
             if (i >= cnt) break;
             if (threadIdx.x == 0) next = atomicAdd(ctr, 1u);  // overlaps this tile
             const int w = base + i;
-            const PairDesc &P = b.pairs[b.tile_pair[w]];
+            const PairDesc &P = b.tiles[w];
             const HsImage im = phase_image(P, ph, b.phase0);
             // per-cell tap addresses: the persistent kernel's own state leaves
             // no registers for the shared ones
@@ -262,7 +265,7 @@ __global__ __launch_bounds__(NT) void hs_multi_phase_kernel(BatchArgs b, Taps ta
     // goes forward, phase 0 backward), so a phase starts on the tiles whose
     // data the previous launch left in that XCD's L2 and in the MALL
     const int w = xcd_remap(blockIdx.x, b.total_tiles, (ph & 1) == 0);
-    const PairDesc &P = b.pairs[b.tile_pair[w]];
+    const PairDesc &P = b.tiles[w];
     const HsImage im = phase_image(P, ph, b.phase0);
     hs_tile<NT, TW, TH, KMAX, NTAP, false, DB, FDIV>(im, w - P.tile_base, kk, tp,
                                                       uvm + Win::PAD);
@@ -322,7 +325,7 @@ struct BatchPlan {
     std::vector<char> key;
     int device = 0;
     int variant = 0, k = 1, grid = 0, total_tiles = 0;
-    PairDesc *d_pairs = nullptr;  // npairs descriptors, then total_tiles pair indices
+    PairDesc *d_pairs = nullptr;  // one pair descriptor per tile of the list
     unsigned *d_sync = nullptr;  // [0] barrier counter, [1] status, claims from [64]
     int64_t sync_words = 0;
     hipGraph_t graph = nullptr;
@@ -353,8 +356,7 @@ static int launch_batch(const BatchPlan &P, const SetupTaps &st, const Taps &av,
     int rc = launch_fill_u32(P.d_sync, 0u, P.sync_words, s);
     if (rc) return rc;
     BatchArgs a;
-    a.pairs = P.d_pairs;
-    a.tile_pair = reinterpret_cast<const int *>(P.d_pairs + npairs);
+    a.tiles = P.d_pairs;
     a.claims = P.d_sync + 64;
     a.barrier = P.d_sync;
     a.status = (int *)(P.d_sync + 1);
@@ -469,17 +471,15 @@ static int build_plan(BatchPlan &P, const std::vector<PairDesc> &desc, int ntaps
     const int tiles = P.total_tiles;
     int rc = resident_grid(P.variant, ntaps, tiles, P.device, P.grid);
     if (rc) return rc;
-    std::vector<int> tile_pair(tiles);
+    std::vector<PairDesc> tile_desc(tiles);
     for (int i = 0; i < npairs; ++i)
-        std::fill(tile_pair.begin() + desc[i].tile_base,
-                  tile_pair.begin() + desc[i].tile_base + desc[i].ntiles, i);
-    HMX_HIP_TRY(hipMalloc(&P.d_pairs, sizeof(PairDesc) * npairs + sizeof(int) * tiles));
+        std::fill(tile_desc.begin() + desc[i].tile_base,
+                  tile_desc.begin() + desc[i].tile_base + desc[i].ntiles, desc[i]);
+    HMX_HIP_TRY(hipMalloc(&P.d_pairs, sizeof(PairDesc) * tiles));
     const int64_t phases = (iters + P.k - 1) / P.k;
     P.sync_words = 64 + (phases + 1) * kXcds * kClaimStride;
     HMX_HIP_TRY(hipMalloc(&P.d_sync, sizeof(unsigned) * P.sync_words));
-    HMX_HIP_TRY(hipMemcpy(P.d_pairs, desc.data(), sizeof(PairDesc) * npairs,
-                          hipMemcpyHostToDevice));
-    HMX_HIP_TRY(hipMemcpy(P.d_pairs + npairs, tile_pair.data(), sizeof(int) * tiles,
+    HMX_HIP_TRY(hipMemcpy(P.d_pairs, tile_desc.data(), sizeof(PairDesc) * tiles,
                           hipMemcpyHostToDevice));
     return HMX_OK;
 }

@@ -353,6 +353,15 @@ extern "C" int hmx_hs_variant_info(int variant, int *tile_w, int *tile_h, int *t
     return HMX_OK;
 }
 
+extern "C" int hmx_hs_config(int H, int W, int *variant, int *k) {
+    HMX_REQUIRE(H > 0 && W > 0, "hmx_hs_config: H and W must be positive");
+    int vi, kk;
+    hs_choice(H, W, vi, kk);
+    if (variant) *variant = vi;
+    if (k) *k = kk;
+    return HMX_OK;
+}
+
 extern "C" int hmx_hs_variant_strip(int variant) {
     HMX_REQUIRE(variant >= 0 && variant < kNumVariants, "unknown HS kernel variant");
     return kVariants[variant].strip;

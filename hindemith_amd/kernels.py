@@ -19,7 +19,8 @@ from ._lib import HindemithError, check, lib
 __all__ = [
     "taps_of", "stencil2d", "dia2d_spmv", "csr_spmv", "CSRBins", "csr_spmv_binned",
     "CSRRowBlocks", "csr_spmv_rowblocks", "CSRSlices", "csr_spmv_sliced",
-    "sum2d", "hs_setup", "hs_jacobi", "hs_solve", "hs_tune", "hs_variants", "stream_ptr",
+    "sum2d", "hs_setup", "hs_jacobi", "hs_solve", "hs_tune", "hs_variants", "hs_config",
+    "stream_ptr",
     "hs_precondition", "hs_matrix_powers", "warp_img2d", "dia_spmv", "brox_set_matrix",
     "brox_set_redblack_matrix", "brox_precondition",
 ]
@@ -260,6 +261,13 @@ def hs_tune(Ix, Iy, It, alpha2, Avg):
     check(lib().hmx_hs_tune(_ptr(Ix), _ptr(Iy), _ptr(It), ctypes.c_float(alpha2), H, W,
                             *ta.args(), _ptr(u), _ptr(v), _ptr(work[0]), _ptr(work[1]),
                             stream_ptr(), ctypes.byref(vo), ctypes.byref(ko)), "hs_tune")
+    return vo.value, ko.value
+
+
+def hs_config(H, W):
+    """(variant, k) the fused Jacobi launches use for an H x W image."""
+    vo, ko = ctypes.c_int(), ctypes.c_int()
+    check(lib().hmx_hs_config(int(H), int(W), ctypes.byref(vo), ctypes.byref(ko)), "hs_config")
     return vo.value, ko.value
 
 

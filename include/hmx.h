@@ -239,6 +239,10 @@ int hmx_hs_variant_info(int variant, int *tile_w, int *tile_h, int *threads, int
 /* Rows per thread per column of a column-strip variant (hs_strip_kernel), 0 for a
  * rectangular-tile variant (hs_tb_kernel), or a negative HMX_ERR_* code. */
 int hmx_hs_variant_strip(int variant);
+/* The (variant, k) hmx_hs_jacobi and hmx_hs_solve use for an H x W image:
+ * the tuned one if hmx_hs_tune ran for that size in this process, else the
+ * default.  No counterpart in the reference. */
+int hmx_hs_config(int H, int W, int *variant, int *k);
 /* Time every (variant, k) candidate on the given buffers (their contents are
  * overwritten) and cache the best for this H x W; returns the choice.  The
  * default configuration is kept unless a candidate is more than 2% faster. */

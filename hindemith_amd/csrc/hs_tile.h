@@ -40,11 +40,14 @@ struct HsTapTable {
     int ox[HS_MAX_TAPS], oy[HS_MAX_TAPS];
     int pos[HS_MAX_TAPS];  // 3x3 window position (oy+1)*3 + (ox+1)
     int nd;
-    __device__ __forceinline__ explicit HsTapTable(const Taps &t) {
-        nd = t.nd;
+    // nfix > 0: the launch's tap count is known to be nfix (a kernel built
+    // for that count), so the table needs no per-tap test and its weights
+    // stay uniform scalars
+    __device__ __forceinline__ explicit HsTapTable(const Taps &t, int nfix = 0) {
+        nd = nfix > 0 ? nfix : t.nd;
 #pragma unroll
         for (int d = 0; d < HS_MAX_TAPS; ++d) {
-            const bool on = d < t.nd;
+            const bool on = d < nd;
             w[d] = on ? t.w[d] : 0.0f;
             ox[d] = on ? t.ox[d] : 0;
             oy[d] = on ? t.oy[d] : 0;
@@ -173,7 +176,8 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
     // Thread tid owns the window cells c = tid + j*NT (j < P), so the LDS
     // address of tap d of any owned pixel is (tid + dl[d]) + j*NT: one
     // per-thread base per tap, the slot a compile-time offset.  pk packs, for
-    // a region cell inside the image, L << 27 | ry << 16 | E << 12 | rx, where
+    // a region cell inside the image, L << 27 | ry << 16 | E << 12 | rx (ry
+    // and rx only in LOUT builds, the one user of them), where
     // L = k - (the cell's distance outside the output tile) is the last step
     // that updates it (the region of step it is the tile grown by k - it, so
     // the cell is updated exactly when it <= L), and E flags the image
@@ -228,7 +232,7 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
                 const int rx = rxs[j], ry = rys[j];
                 const int dx = max(max(k - rx, rx - (k + TW - 1)), 0);
                 const int dy = max(max(k - ry, ry - (k + TH - 1)), 0);
-                pk[j] = ((k - max(dx, dy)) << 27) | (ry << 16) | rx;
+                pk[j] = ((k - max(dx, dy)) << 27) | (LOUT ? (ry << 16) | rx : 0);
                 if (!pd) {
                     const float m1 = cx[j] * cx[j];
                     const float s1 = a.alpha2 + m1;
@@ -288,7 +292,8 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
                 const int dy = max(max(y0 - gy, gy - y1 + 1), 0);
                 const int e = (gx == 0) | (gx == a.W - 1) << 1 | (gy == 0) << 2 |
                               (gy == a.H - 1) << 3;
-                pk[j] = ((k - max(dx, dy)) << 27) | (rys[j] << 16) | (e << 12) | rxs[j];
+                pk[j] = ((k - max(dx, dy)) << 27) | (e << 12) |
+                        (LOUT ? (rys[j] << 16) | rxs[j] : 0);
                 if (!bd) {
                     const float m1 = cx[j] * cx[j];
                     const float s1 = a.alpha2 + m1;
@@ -305,8 +310,12 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
     // One Jacobi step.  PAR (compile time, SHARED builds) is the window step
     // it reads with DB, (it - 1) & 1, so both windows' tap addresses are
     // constants; other builds pass 0 and select the window at run time.
-    auto step = [&](const int it, auto par_c) {
+    // CLIP (compile time): the tile has ghost cells to refresh.  Unclipped
+    // tiles run a loop without the ghost stores, whose per-cell border masks
+    // would otherwise be set up for every tile.
+    auto step = [&](const int it, auto par_c, auto clip_c) {
         constexpr int PAR = decltype(par_c)::value;
+        constexpr bool CLIP = decltype(clip_c)::value;
         const int act_min = it << 27;
         const int par = SHARED ? PAR : (it - 1) & 1;
         const float2 *uvr = DB ? uv + par * NPX : uv;
@@ -346,7 +355,7 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
                     if (DB) {
                         const float2 q = make_float2(nu[j], nv[j]);
                         uvw[tid + j * NT] = q;
-                        if (clipped) hs_ghost_store<PITCH>(uvw, tid + j * NT, pk[j], q);
+                        if (CLIP) hs_ghost_store<PITCH>(uvw, tid + j * NT, pk[j], q);
                     }
                 }
 #pragma unroll
@@ -377,7 +386,7 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
                 if (DB) {
                     const float2 q = make_float2(nu[j], nv[j]);
                     uvw[tid + j * NT] = q;
-                    if (clipped) hs_ghost_store<PITCH>(uvw, tid + j * NT, pkj, q);
+                    if (CLIP) hs_ghost_store<PITCH>(uvw, tid + j * NT, pkj, q);
                 }
                 if (LOUT) {
                     const int rx = pkj & 0xfff, ry = (pkj >> 16) & 0x7ff;
@@ -404,7 +413,7 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
             if (act[j]) {
                 const float2 q = make_float2(nu[j], nv[j]);
                 uv[tid + j * NT] = q;
-                if (clipped) hs_ghost_store<PITCH>(uv, tid + j * NT, pk[j], q);
+                if (CLIP) hs_ghost_store<PITCH>(uv, tid + j * NT, pk[j], q);
             }
         }
         if (LOUT) {
@@ -423,30 +432,256 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
         }
         __syncthreads();
     };
-    if (DB && SHARED) {
-        for (int it = 1; it <= k; it += 2) {
-            step(it, std::integral_constant<int, 0>());
-            if (it < k) step(it + 1, std::integral_constant<int, 1>());
+    auto run = [&](auto clip_c) {
+        if (DB && SHARED) {
+            for (int it = 1; it <= k; it += 2) {
+                step(it, std::integral_constant<int, 0>(), clip_c);
+                if (it < k) step(it + 1, std::integral_constant<int, 1>(), clip_c);
+            }
+        } else {
+            for (int it = 1; it <= k; ++it) step(it, std::integral_constant<int, 0>(), clip_c);
         }
-    } else {
-        for (int it = 1; it <= k; ++it) step(it, std::integral_constant<int, 0>());
-    }
+    };
+    if (clipped)
+        run(std::true_type());
+    else
+        run(std::false_type());
 
+    // Store the output tile from the final window (complete: every step ends
+    // with a barrier) in tile order: thread tid stores tile cells tid + j*NT,
+    // so a wavefront writes whole row segments, at byte offsets from the
+    // tile's corner (32 bits: H, W < 32768).
+    {
+        const float2 *const fin = (DB ? uv + (k & 1) * NPX : uv) + k * PITCH + k;
+        const int64_t corner = ((int64_t)y0 * a.W + x0) * (int64_t)sizeof(float);
+        char *const pu = reinterpret_cast<char *>(a.u_out) + corner;
+        char *const pv = reinterpret_cast<char *>(a.v_out) + corner;
+        const unsigned rowb = (unsigned)a.W * (unsigned)sizeof(float);
+        const int tw = x1 - x0, th = y1 - y0;
+        constexpr int NO = (TW * TH + NT - 1) / NT;
+        // all of a thread's cells are read before the first store, so no
+        // store waits for an earlier one to leave the registers it reads
+        float2 q[NO];
+        unsigned o[NO];
+        bool st[NO];
+#pragma unroll
+        for (int j = 0; j < NO; ++j) {
+            const int i = tid + j * NT;
+            const int ty = i / TW, tx = i - ty * TW;
+            st[j] = ((j + 1) * NT <= TW * TH || i < TW * TH) && tx < tw && ty < th;
+            o[j] = (unsigned)ty * rowb + (unsigned)tx * (unsigned)sizeof(float);
+            if (st[j]) q[j] = fin[ty * PITCH + tx];
+        }
+#pragma unroll
+        for (int j = 0; j < NO; ++j) {
+            if (st[j]) {
+                *reinterpret_cast<float *>(pu + o[j]) = q[j].x;
+                *reinterpret_cast<float *>(pv + o[j]) = q[j].y;
+            }
+        }
+    }
+    __syncthreads();  // the window is reused by the work-group's next tile
+}
+
+
+// Ring-ordered tile body for 64 x 32 tiles, k <= 8, double-buffered window,
+// the 8-tap averaging stencil at the canonical offsets (HS_RING_OX/OY, in
+// that tap order; its weights stay run-time values).
+//
+// The cells a Jacobi step updates are the output tile grown by k - it, so a
+// cell's last step is k - (its distance from the tile).  hs_tile gives thread
+// tid the window cells tid + j*NT in row-major order; a wavefront's 64 cells
+// then straddle rows and distances, and 12 % of the lanes of the wave-slots
+// that run are idle.  Here cells are owned by distance instead: wave-slot S
+// (wave S % 16 of the work-group, cell slot S / 16 of each of its threads)
+// holds core row S of the tile (S < 32, last step k) or a quarter of the ring
+// at distance m (S = 32 + 4(m - 1) .. +3, last step k - m), so nearly every
+// lane of a running wave-slot is active: 368 wave-slot steps per tile at
+// k = 8 instead of 396.  A cell's LDS address is then its own register and
+// its taps are compile-time offsets from it, which is why the stencil offsets
+// are fixed.  (u, v) of the whole region are still loaded in row-major order
+// (coalesced); each owner loads its own coefficients.  Same per-pixel
+// arithmetic, tap order, ghost cells and results as hs_tile.
+// The window's row pitch is odd (81 float2 = 162 dwords): the wave-slots on
+// the left and right rings hold 64 cells of one column, and with the even
+// pitch 80 (160 dwords, 32 mod the 64 LDS banks) every lane of such a read
+// fell on the same two bank pairs.  At 162 dwords the 16 lanes served in one
+// LDS cycle touch 16 distinct bank pairs.
+constexpr int HS_RING_OX[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
+constexpr int HS_RING_OY[8] = {0, 0, -1, 1, -1, -1, 1, 1};
+constexpr int HS_RING_PITCH = 64 + 2 * 8 + 1;
+constexpr int HS_RING_NPX = HS_RING_PITCH * (32 + 2 * 8);
+// LDS of the ring form's two windows, with PITCH + 1 cells before and after
+// them for the taps of spare lanes (the kernel passes the allocation + PAD)
+constexpr int HS_RING_PAD = HS_RING_PITCH + 1;
+constexpr int HS_RING_LDS = HS_RING_PAD + 2 * HS_RING_NPX + HS_RING_PAD;
+
+__host__ __device__ inline bool hs_ring_taps(const Taps &t) {
+    if (t.nd != 8) return false;
+    for (int d = 0; d < 8; ++d)
+        if (t.ox[d] != HS_RING_OX[d] || t.oy[d] != HS_RING_OY[d]) return false;
+    return true;
+}
+
+template <bool FDIV>
+__device__ __forceinline__ void hs_tile_ring(const HsImage &a, int t, int k,
+                                             const HsTapTable<HS_RING_PITCH> &tp, float2 *uv) {
+    constexpr int NT = 1024, TW = 64, TH = 32, KMAX = 8;
+    constexpr int PITCH = HS_RING_PITCH, NPX = HS_RING_NPX;
+    constexpr int P = 4;  // cell slots per thread: 32 core rows + 4 per ring, <= 64 slots
+    static_assert(NPX <= P * NT, "window larger than the loader pass");
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tyi = t / a.tiles_x, txi = t - tyi * a.tiles_x;
+    const int x0 = txi * TW, y0 = tyi * TH;
+    const int x1 = min(x0 + TW, a.W), y1 = min(y0 + TH, a.H);
+    const int gx0 = x0 - k, gy0 = y0 - k;
+    const int RW = x1 - x0 + 2 * k, RH = y1 - y0 + 2 * k;
+    const bool clipped = (gx0 < 0) || (gy0 < 0) || (x1 + k > a.W) || (y1 + k > a.H);
+
+    // Byte offsets from the image start fit 32 bits (H, W < 32768): every
+    // load below is a uniform base plus one offset register.
+    const unsigned rowb = (unsigned)a.W * (unsigned)sizeof(float);
+    const char *const bIx = reinterpret_cast<const char *>(a.Ix);
+    const char *const bIy = reinterpret_cast<const char *>(a.Iy);
+    const char *const bIt = reinterpret_cast<const char *>(a.It);
+    const char *const bd = reinterpret_cast<const char *>(a.den);
+    const char *const bu = reinterpret_cast<const char *>(a.u_in);
+    const char *const bv = reinterpret_cast<const char *>(a.v_in);
+
+    // (u, v) of every region cell into window 0, row-major (coalesced); cells
+    // outside the image load the pixel they clamp to (ghosts, see hs_tile)
 #pragma unroll
     for (int j = 0; j < P; ++j) {
-        if (pk[j] < 0) continue;
-        const int rx = pk[j] & 0xfff, ry = (pk[j] >> 16) & 0x7ff;
+        const int c = tid + j * NT;
+        const int ry = c / PITCH, rx = c - ry * PITCH;
+        if (((j + 1) * NT <= NPX || c < NPX) && rx < RW && ry < RH) {
+            const unsigned o = (unsigned)clampi(gy0 + ry, 0, a.H - 1) * rowb +
+                               (unsigned)clampi(gx0 + rx, 0, a.W - 1) * (unsigned)sizeof(float);
+            uv[c] = make_float2(*reinterpret_cast<const float *>(bu + o),
+                                *reinterpret_cast<const float *>(bv + o));
+        }
+    }
+
+    // owned cells: window position, last step, border flags, coefficients.
+    // pk as in hs_tile (last step << 27 | ry << 16 | border flags << 12 | rx;
+    // -1: not updated); ca = the cell's window index (0 for a spare lane, whose
+    // taps then stay inside the padded allocation and are never used).  Slot
+    // S's ring distance and quarter are uniform over the wave; only the
+    // position along the ring is per lane.
+    float cx[P], cy[P], ct[P], cd[P], cr[FDIV ? P : 1];
+    int pk[P], ca[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        const int S = wave + 16 * j;
+        int rx, ry, L;
+        if (S < TH) {  // core row S
+            rx = k + lane;
+            ry = k + S;
+            L = k;
+        } else {
+            const int m = (S - TH) / 4 + 1;  // ring distance (uniform)
+            const int i = ((S - TH) & 3) * 64 + lane;
+            const int top = TW + 2 * m, side = TH + 2 * m - 2;
+            const int i1 = i - top, i2 = i - 2 * top, i3 = i2 - side;
+            L = (m < k && i3 < side) ? k - m : -1;
+            rx = i < top ? k - m + i : i1 < top ? k - m + i1 : i3 < 0 ? k - m : k + TW - 1 + m;
+            ry = i < top ? k - m : i1 < top ? k + TH - 1 + m : k - m + 1 + (i3 < 0 ? i2 : i3);
+        }
         const int gx = gx0 + rx, gy = gy0 + ry;
-        if (gx >= x0 && gx < x1 && gy >= y0 && gy < y1) {
+        // an unclipped tile's cells are all image pixels off the border
+        const bool img =
+            L >= 0 && (!clipped || ((unsigned)gx < (unsigned)a.W && (unsigned)gy < (unsigned)a.H));
+        ca[j] = L >= 0 ? ry * PITCH + rx : 0;
+        pk[j] = -1;
+        if (img) {
+            const int e = !clipped ? 0
+                                   : (gx == 0) | (gx == a.W - 1) << 1 | (gy == 0) << 2 |
+                                         (gy == a.H - 1) << 3;
+            pk[j] = (L << 27) | (e << 12);
+            const unsigned o = (unsigned)gy * rowb + (unsigned)gx * (unsigned)sizeof(float);
+            cx[j] = *reinterpret_cast<const float *>(bIx + o);
+            cy[j] = *reinterpret_cast<const float *>(bIy + o);
+            ct[j] = *reinterpret_cast<const float *>(bIt + o);
+            if (bd) cd[j] = *reinterpret_cast<const float *>(bd + o);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        if (FDIV) cr[j] = 1.0f;
+        if (pk[j] >= 0) {
+            if (!bd) {
+                const float m1 = cx[j] * cx[j];
+                const float s1 = a.alpha2 + m1;
+                const float m2 = cy[j] * cy[j];
+                cd[j] = s1 + m2;
+            }
+            if (FDIV) cr[j] = hs_recip(cd[j]);
+        }
+    }
+    __syncthreads();
+
+    // One Jacobi step reading window PAR and writing the other; the first
+    // taps of the next cell are read before the current one is computed.
+    auto step = [&](const int it, auto par_c) {
+        constexpr int PAR = decltype(par_c)::value;
+        const int act_min = it << 27;
+        const float2 *uvr = uv + PAR * NPX;
+        float2 *uvw = uv + (PAR ^ 1) * NPX;
+        constexpr int NPF = FDIV ? 2 : 4;
+        float2 qa[NPF];
+#pragma unroll
+        for (int d = 0; d < NPF; ++d) qa[d] = uvr[ca[0] + HS_RING_OY[d] * PITCH + HS_RING_OX[d]];
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            float2 qr[8 - NPF], qn[NPF];
+#pragma unroll
+            for (int d = NPF; d < 8; ++d)
+                qr[d - NPF] = uvr[ca[j] + HS_RING_OY[d] * PITCH + HS_RING_OX[d]];
+            if (j + 1 < P) {
+#pragma unroll
+                for (int d = 0; d < NPF; ++d)
+                    qn[d] = uvr[ca[j + 1] + HS_RING_OY[d] * PITCH + HS_RING_OX[d]];
+            }
+            if (pk[j] >= act_min) {
+                float ub = 0.0f, vb = 0.0f;
+#pragma unroll
+                for (int d = 0; d < 8; ++d) {
+                    const float2 qd = d < NPF ? qa[d] : qr[d - NPF];
+                    ub = fmaf(tp.w[d], qd.x, ub);
+                    vb = fmaf(tp.w[d], qd.y, vb);
+                }
+                float nu, nv;
+                (void)hs_update<FDIV>(cx[j], cy[j], ct[j], cd[j], FDIV ? cr[j] : 0.0f, ub, vb,
+                                      nu, nv);
+                const float2 q = make_float2(nu, nv);
+                uvw[ca[j]] = q;
+                if (clipped) hs_ghost_store<PITCH>(uvw, ca[j], pk[j], q);
+            }
+#pragma unroll
+            for (int d = 0; d < NPF; ++d) qa[d] = qn[d];
+        }
+        __syncthreads();
+    };
+    for (int it = 1; it <= k; it += 2) {
+        step(it, std::integral_constant<int, 0>());
+        if (it < k) step(it + 1, std::integral_constant<int, 1>());
+    }
+
+    // the core rows' owners store the output tile from the last window
+    const float2 *fin = uv + (k & 1) * NPX;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int S = wave + 16 * j;
+        const int gx = x0 + lane, gy = y0 + S;
+        if (gx < x1 && gy < y1) {
             const int64_t g = (int64_t)gy * a.W + gx;
-            const float2 q = (DB ? uv + (k & 1) * NPX : uv)[tid + j * NT];
+            const float2 q = fin[(k + S) * PITCH + k + lane];
             a.u_out[g] = q.x;
             a.v_out[g] = q.y;
         }
     }
     __syncthreads();  // the window is reused by the work-group's next tile
 }
-
 
 // Column-strip tile body.  Thread q of the work-group owns S vertically
 // adjacent pixels of one column of the region (strips are laid out column-

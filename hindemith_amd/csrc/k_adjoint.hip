@@ -203,7 +203,7 @@ __global__ __launch_bounds__(NT, 8) void hs_adj_tb_kernel(AdjTbArgs a, Taps taps
         }
     }
     auto in_reach = [](int pkj, int e) { return (unsigned)pkj < ((unsigned)e + 1u) << 27; };
-    const HsTapTable<PITCH> tp(taps);
+    const HsTapTable<PITCH> tp(taps, NTAP);
 
     const float *Ix = a.Ix + off, *Iy = a.Iy + off;
     float cx[P], cy[P], cd[P];

@@ -197,7 +197,7 @@ __global__ __launch_bounds__(NT, WPE) void hs_batched_kernel(BatchArgs b, SetupT
     __shared__ float2 uvm[Win::SIZE];
     float2 *const uv = uvm + Win::PAD;
     __shared__ int s_claim;
-    const HsTapTable<TW + 2 * KMAX> tp(taps);
+    const HsTapTable<TW + 2 * KMAX> tp(taps, NTAP);
     unsigned arrivals = 0;
     const int xcd = blockIdx.x % kXcds, slot = blockIdx.x / kXcds;
     const int per_xcd = gridDim.x / kXcds;  // the host launches a multiple of kXcds
@@ -260,7 +260,7 @@ __global__ __launch_bounds__(NT) void hs_multi_phase_kernel(BatchArgs b, Taps ta
                                                             int kk) {
     using Win = HsWindow<NT, TW, TH, KMAX, DB>;
     __shared__ float2 uvm[Win::SIZE];
-    const HsTapTable<TW + 2 * KMAX> tp(taps);
+    const HsTapTable<TW + 2 * KMAX> tp(taps, NTAP);
     // phases alternate the walk over each XCD's share of the list (the setup
     // goes forward, phase 0 backward), so a phase starts on the tiles whose
     // data the previous launch left in that XCD's L2 and in the MALL

@@ -74,9 +74,32 @@ __global__ __launch_bounds__(NT, LOUT ? 8 : 1) void hs_tb_kernel(HsArgs a, Taps 
         im.lvb = a.lvb + off;
         im.lstride = a.lstride;
     }
-    const HsTapTable<TW + 2 * KMAX> tp(taps);
+    const HsTapTable<TW + 2 * KMAX> tp(taps, NTAP);
     hs_tile<NT, TW, TH, KMAX, NTAP, LOUT, DB, FDIV>(im, xcd_remap(blockIdx.x, a.ntiles, a.rev), a.k,
                                                     tp, uvm + Win::PAD, LOUT ? lin : nullptr);
+}
+
+// Ring-ordered form (hs_tile_ring): 64 x 32 tiles, k <= 8, double-buffered,
+// the canonical 8-tap averaging offsets.
+template <bool FDIV>
+__global__ __launch_bounds__(1024, 8) void hs_ring_kernel(HsArgs a, Taps taps) {
+    __shared__ float2 uvm[HS_RING_LDS];
+    const int64_t off = (int64_t)blockIdx.y * a.bstride;
+    HsImage im;
+    im.Ix = a.Ix + off;
+    im.Iy = a.Iy + off;
+    im.It = a.It + off;
+    im.den = a.den ? a.den + off : nullptr;
+    im.u_in = a.u_in + off;
+    im.v_in = a.v_in + off;
+    im.u_out = a.u_out + off;
+    im.v_out = a.v_out + off;
+    im.alpha2 = a.alpha2_b ? a.alpha2_b[blockIdx.y] : a.alpha2;
+    im.H = a.H;
+    im.W = a.W;
+    im.tiles_x = a.tiles_x;
+    const HsTapTable<HS_RING_PITCH> tp(taps, 8);
+    hs_tile_ring<FDIV>(im, xcd_remap(blockIdx.x, a.ntiles, a.rev), a.k, tp, uvm + HS_RING_PAD);
 }
 
 template <int NT, int TW, int TH, int KMAX, int S, int NTAP>
@@ -96,7 +119,7 @@ __global__ __launch_bounds__(NT) void hs_strip_kernel(HsArgs a, Taps taps) {
     im.H = a.H;
     im.W = a.W;
     im.tiles_x = a.tiles_x;
-    const HsTapTable<TW + 2 * KMAX> tp(taps);
+    const HsTapTable<TW + 2 * KMAX> tp(taps, NTAP);
     hs_tile_strip<NT, TW, TH, KMAX, S, NTAP>(im, xcd_remap(blockIdx.x, a.ntiles, a.rev), a.k, tp,
                                              uv);
 }
@@ -108,6 +131,7 @@ struct HsVariant {
     int nt, tw, th, kmax;
     int strip;                        // rows per thread per column (0: rectangular tile)
     void (*kernel[5])(HsArgs, Taps);  // by tap class: 4, 5, 8, 9 taps, any
+    void (*ring)(HsArgs, Taps);       // ring-ordered kernel for the canonical 8 taps, or null
 };
 
 // tap count -> kernel slot (straight-line kernels for the common stencils)
@@ -125,13 +149,13 @@ static int tap_class(int nd) {
     {NT, TW, TH, K, 0,                                                                  \
      {hs_tb_kernel<NT, TW, TH, K, 4>, hs_tb_kernel<NT, TW, TH, K, 5>,                  \
       hs_tb_kernel<NT, TW, TH, K, 8>, hs_tb_kernel<NT, TW, TH, K, 9>,                  \
-      hs_tb_kernel<NT, TW, TH, K, 0>}}
+      hs_tb_kernel<NT, TW, TH, K, 0>}, nullptr}
 // double-buffered window, one barrier per step
 #define HS_D(NT, TW, TH, K)                                                             \
     {NT, TW, TH, K, 0,                                                                  \
      {hs_tb_kernel<NT, TW, TH, K, 4, false, true>, hs_tb_kernel<NT, TW, TH, K, 5, false, true>, \
       hs_tb_kernel<NT, TW, TH, K, 8, false, true>, hs_tb_kernel<NT, TW, TH, K, 9, false, true>, \
-      hs_tb_kernel<NT, TW, TH, K, 0, false, true>}}
+      hs_tb_kernel<NT, TW, TH, K, 0, false, true>}, nullptr}
 // quotient from a per-pixel reciprocal (hs_div.h): single or double-buffered window
 #define HS_F(NT, TW, TH, K, DB)                                                         \
     {NT, TW, TH, K, 0,                                                                  \
@@ -139,13 +163,13 @@ static int tap_class(int nd) {
       hs_tb_kernel<NT, TW, TH, K, 5, false, DB, true>,                                  \
       hs_tb_kernel<NT, TW, TH, K, 8, false, DB, true>,                                  \
       hs_tb_kernel<NT, TW, TH, K, 9, false, DB, true>,                                  \
-      hs_tb_kernel<NT, TW, TH, K, 0, false, DB, true>}}
+      hs_tb_kernel<NT, TW, TH, K, 0, false, DB, true>}, nullptr}
 // column-strip variants (S pixels per thread per column)
 #define HS_S(NT, TW, TH, K, S)                                                          \
     {NT, TW, TH, K, S,                                                                  \
      {hs_strip_kernel<NT, TW, TH, K, S, 4>, hs_strip_kernel<NT, TW, TH, K, S, 5>,      \
       hs_strip_kernel<NT, TW, TH, K, S, 8>, hs_strip_kernel<NT, TW, TH, K, S, 9>,      \
-      hs_strip_kernel<NT, TW, TH, K, S, 0>}}
+      hs_strip_kernel<NT, TW, TH, K, S, 0>}, nullptr}
 static const HsVariant kVariants[] = {
     HS_V(256, 64, 16, 1),     // 0: one iteration per launch (plain sweep)
     HS_V(256, 64, 32, 4),     // 1
@@ -174,6 +198,15 @@ static const HsVariant kVariants[] = {
     HS_F(1024, 64, 32, 8, true),   // 24: reciprocal quotient, double-buffered
     HS_F(1024, 64, 32, 4, true),   // 25
     HS_F(512, 64, 32, 4, true),    // 26
+    // 27: as 24, with ring-ordered cell ownership for the canonical 8-tap
+    // averaging stencil (other stencils run variant 24's kernels)
+    {1024, 64, 32, 8, 0,
+     {hs_tb_kernel<1024, 64, 32, 8, 4, false, true, true>,
+      hs_tb_kernel<1024, 64, 32, 8, 5, false, true, true>,
+      hs_tb_kernel<1024, 64, 32, 8, 8, false, true, true>,
+      hs_tb_kernel<1024, 64, 32, 8, 9, false, true, true>,
+      hs_tb_kernel<1024, 64, 32, 8, 0, false, true, true>},
+     hs_ring_kernel<true>},
 };
 #undef HS_V
 #undef HS_D
@@ -224,6 +257,7 @@ int launch_hs_tb(int variant, const float *Ix, const float *Iy, const float *It,
     const int tiles_y = (H + V.th - 1) / V.th;
     a.ntiles = a.tiles_x * tiles_y;
     auto kern = V.kernel[tap_class(taps.nd)];
+    if (V.ring && hs_ring_taps(taps)) kern = V.ring;
     if (lt || lub) {
         // linearisation output exists only for the 1024-thread 64x32 k<=8 shape
         static void (*const kLinKernels[5])(HsArgs, Taps) = {

@@ -63,6 +63,7 @@ _SIGS = {
     "hmx_hs_variant_info": [c_int, pint, pint, pint, pint],
     "hmx_hs_variant_strip": [c_int],
     "hmx_hs_config": [c_int, c_int, pint, pint],
+    "hmx_hs_recip_check": [c_int, c_int, vp, vp],
     "hmx_hs_tune": [vp, vp, vp, c_float, c_int, c_int, c_int, vp, vp, vp, vp, vp, vp, vp,
                     vp, pint, pint],
     "hmx_hs_solve": [vp, vp, c_int, c_int, c_float] + [c_int, vp, vp, vp] * 4 +

@@ -20,6 +20,15 @@
 // and hs_div_fast reports failure; a failed lane (also a zero, tiny, huge or
 // non-finite quotient) takes the IEEE division instead.
 //
+// y itself: on the GPU, y = RN(y0 + RN(1 - b*y0)*y0) from the hardware
+// reciprocal y0 = v_rcp_f32(b) (within one ulp), three instructions instead
+// of the ten of the division sequence.  One such Newton step from a faithful
+// reciprocal is correctly rounded except possibly when b's significand is all
+// ones (Markstein's reciprocal theorem, Theorem 4.9 of the same Handbook); for
+// this hardware reciprocal it equals the IEEE 1/b for every b in the accepted
+// range, which hmx_hs_recip_check verifies exhaustively on the device (every
+// significand, every exponent, both signs).  The host computes 1/b.
+//
 // Plain C++ as well as HIP, so the CPU test can check it against the host's
 // IEEE division on hundreds of millions of operand pairs.
 #pragma once
@@ -36,7 +45,13 @@ namespace hmx {
 
 HS_DIV_HD inline float hs_recip(float b) {
     const float ab = std::fabs(b);
-    return (ab >= 0x1p-24f && ab <= 0x1p24f) ? 1.0f / b : __builtin_nanf("");
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float y0 = __builtin_amdgcn_rcpf(b);
+    const float y = std::fma(std::fma(-b, y0, 1.0f), y0, y0);
+#else
+    const float y = 1.0f / b;
+#endif
+    return (ab >= 0x1p-24f && ab <= 0x1p24f) ? y : __builtin_nanf("");
 }
 
 // q = RN(a / b) when the result is true; otherwise q is unspecified and the

@@ -108,6 +108,11 @@ __device__ __forceinline__ void hs_ghost_store(float2 *w, int c, int pk, float2 
     }
 }
 
+// |a - b| + c on unsigned values, the form the compiler emits as v_sad_u32
+__device__ __forceinline__ unsigned hs_usad(unsigned a, unsigned b, unsigned c) {
+    return (max(a, b) - min(a, b)) + c;
+}
+
 // LDS of one tile window (two with DB): PAD cells before and after it, so a
 // thread may read the taps of any cell it owns, updated or not, without
 // leaving the allocation.  hs_tile takes uv = the allocation + PAD.
@@ -230,9 +235,14 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
             pk[j] = -1;
             if (in[j]) {
                 const int rx = rxs[j], ry = rys[j];
-                const int dx = max(max(k - rx, rx - (k + TW - 1)), 0);
-                const int dy = max(max(k - ry, ry - (k + TH - 1)), 0);
-                pk[j] = ((k - max(dx, dy)) << 27) | (LOUT ? (ry << 16) | rx : 0);
+                // twice the cell's distance outside the tile along an axis,
+                // from its distance to the tile's centre line (at 2k + T - 1
+                // in doubled coordinates): |2r - (2k + T - 1)| - (T - 1), one
+                // v_sad_u32 per axis
+                const int ax = (int)hs_usad(2u * rx, 2u * k + (TW - 1), 0u - (TW - 1));
+                const int ay = (int)hs_usad(2u * ry, 2u * k + (TH - 1), 0u - (TH - 1));
+                const int m2 = max(max(ax, ay), 0);  // 2 x (distance outside)
+                pk[j] = ((2 * k - m2) << 26) | (LOUT ? (ry << 16) | rx : 0);
                 if (!pd) {
                     const float m1 = cx[j] * cx[j];
                     const float s1 = a.alpha2 + m1;

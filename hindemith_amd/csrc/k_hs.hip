@@ -370,6 +370,29 @@ static int validate_jacobi(const float *Ix, const float *Iy, const float *It, in
     return HMX_OK;
 }
 
+// Exhaustive check of hs_recip on the device: every b = +-(1.m) * 2^e with e in
+// [e_lo, e_hi] against the correctly rounded 1.0f / b (NaN outside the range
+// hs_recip accepts); mismatches are added to *bad.
+__global__ __launch_bounds__(256) void hs_recip_check_kernel(int e_lo, int e_hi,
+                                                             unsigned long long *bad) {
+    const uint64_t n = (uint64_t)(e_hi - e_lo + 1) << 24;  // 2 signs x 2^23 significands
+    unsigned long long cnt = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t m = (uint32_t)i & 0x7fffffu;
+        const uint32_t sign = (uint32_t)(i >> 23) & 1u;
+        const int e = e_lo + (int)(i >> 24);
+        const float b = __uint_as_float(sign << 31 | (uint32_t)(e + 127) << 23 | m);
+        const float ab = fabsf(b);
+        const bool in = ab >= 0x1p-24f && ab <= 0x1p24f;
+        const float ref = in ? 1.0f / b : __builtin_nanf("");
+        const float got = hs_recip(b);
+        const bool same = in ? __float_as_uint(got) == __float_as_uint(ref) : got != got;
+        cnt += same ? 0 : 1;
+    }
+    if (cnt) atomicAdd(bad, cnt);
+}
+
 }  // namespace hmx
 
 using namespace hmx;
@@ -384,6 +407,16 @@ extern "C" int hmx_hs_variant_info(int variant, int *tile_w, int *tile_h, int *t
     if (tile_h) *tile_h = V.th;
     if (threads) *threads = V.nt;
     if (kmax) *kmax = V.kmax;
+    return HMX_OK;
+}
+
+extern "C" int hmx_hs_recip_check(int e_lo, int e_hi, unsigned long long *bad, void *stream) {
+    HMX_REQUIRE(bad, "hmx_hs_recip_check: null counter");
+    HMX_REQUIRE(e_lo >= -126 && e_hi <= 127 && e_lo <= e_hi,
+                "hmx_hs_recip_check: exponents must satisfy -126 <= e_lo <= e_hi <= 127");
+    hipLaunchKernelGGL(hs_recip_check_kernel, dim3(8192), dim3(256), 0, as_stream(stream), e_lo,
+                       e_hi, bad);
+    HMX_LAUNCH_CHECK("hs_recip_check_kernel");
     return HMX_OK;
 }
 

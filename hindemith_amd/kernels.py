@@ -20,7 +20,7 @@ __all__ = [
     "taps_of", "stencil2d", "dia2d_spmv", "csr_spmv", "CSRBins", "csr_spmv_binned",
     "CSRRowBlocks", "csr_spmv_rowblocks", "CSRSlices", "csr_spmv_sliced",
     "sum2d", "hs_setup", "hs_jacobi", "hs_solve", "hs_tune", "hs_variants", "hs_config",
-    "stream_ptr",
+    "hs_recip_check", "stream_ptr",
     "hs_precondition", "hs_matrix_powers", "warp_img2d", "dia_spmv", "brox_set_matrix",
     "brox_set_redblack_matrix", "brox_precondition",
 ]
@@ -269,6 +269,16 @@ def hs_config(H, W):
     vo, ko = ctypes.c_int(), ctypes.c_int()
     check(lib().hmx_hs_config(int(H), int(W), ctypes.byref(vo), ctypes.byref(ko)), "hs_config")
     return vo.value, ko.value
+
+
+def hs_recip_check(e_lo=-24, e_hi=24):
+    """Number of b = +-(1.m) * 2^e, e_lo <= e <= e_hi, for which the device
+    reciprocal of the reciprocal quotient differs from the correctly rounded
+    1/b (NaN outside [2^-24, 2^24]).  Zero means the quotient's y = RN(1/b)."""
+    bad = torch.zeros(1, dtype=torch.int64, device="cuda")
+    check(lib().hmx_hs_recip_check(int(e_lo), int(e_hi), ctypes.c_void_p(bad.data_ptr()),
+                                   stream_ptr()), "hs_recip_check")
+    return int(bad.item())
 
 
 def hs_variants():

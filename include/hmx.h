@@ -239,6 +239,14 @@ int hmx_hs_variant_info(int variant, int *tile_w, int *tile_h, int *threads, int
 /* Rows per thread per column of a column-strip variant (hs_strip_kernel), 0 for a
  * rectangular-tile variant (hs_tb_kernel), or a negative HMX_ERR_* code. */
 int hmx_hs_variant_strip(int variant);
+/* Self-check of the per-pixel reciprocal the fused kernel's reciprocal
+ * quotient starts from (on the device: the hardware reciprocal and one Newton
+ * step): every b = +-(1.m) * 2^e, e_lo <= e <= e_hi, is compared with the
+ * correctly rounded 1/b (NaN outside [2^-24, 2^24]), and the number of
+ * mismatches is added to *bad (a device counter the caller zeroes).  Enqueued
+ * on stream.  No counterpart in the reference: it pins the bit-exactness of
+ * the quotient the reference computes with a division. */
+int hmx_hs_recip_check(int e_lo, int e_hi, unsigned long long *bad, void *stream);
 /* The (variant, k) hmx_hs_jacobi and hmx_hs_solve use for an H x W image:
  * the tuned one if hmx_hs_tune ran for that size in this process, else the
  * default.  No counterpart in the reference. */

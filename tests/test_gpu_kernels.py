@@ -343,6 +343,17 @@ def _bitwise_equal_nan(got, ref):
             and np.array_equal(got[~nan].view(np.uint32), ref[~nan].view(np.uint32)))
 
 
+def test_device_reciprocal_exhaustive(K):
+    """The per-pixel reciprocal the reciprocal quotient starts from is computed on
+    the device from the hardware reciprocal and one Newton step; Markstein's
+    theorem needs it to equal RN(1/b).  Every significand, both signs, every
+    exponent of the accepted range [2^-24, 2^24] (822 M values) against the
+    device's correctly rounded division, and NaN for the exponents just outside."""
+    assert K.hs_recip_check(-24, 24) == 0
+    assert K.hs_recip_check(-40, -25) == 0
+    assert K.hs_recip_check(25, 40) == 0
+
+
 def test_hs_jacobi_division_edge_cases(K):
     """Operands the reciprocal quotient must hand to the IEEE division: zero and
     negative-zero numerators, denominators outside [2^-24, 2^24] (tiny alpha^2

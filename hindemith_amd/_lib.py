@@ -41,6 +41,7 @@ _SIGS = {
     "hmx_result_info": [c_int, c_int, pint, pint, pint, pint64, ctypes.POINTER(c_char_p)],
     "hmx_result_read": [c_int, c_int, vp, c_int64],
     "hmx_plan_text": [c_int, ctypes.c_char_p, c_int64],
+    "hmx_program_source": [c_int, c_int, ctypes.c_char_p, c_int64],
     "hmx_release": [c_int],
     "hmx_stencil2d": [vp, vp, c_int, c_int, c_int, vp, vp, vp, vp],
     "hmx_dia2d_spmv": [vp, c_int, vp, vp, vp, vp, c_int, c_int, vp],

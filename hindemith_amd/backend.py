@@ -28,7 +28,7 @@ import numpy as np
 
 from ._lib import HindemithError, check, lib
 
-__all__ = ["xmlcompile", "run", "free_ocl", "init", "plan"]
+__all__ = ["xmlcompile", "run", "free_ocl", "init", "plan", "program_source"]
 
 _lock = threading.Lock()
 _symtabs = {}    # record id -> {name: (tag, attrs)}
@@ -62,6 +62,15 @@ def xmlcompile(record_id, symtab_xml, sm_xml, defines_xml):
 def plan(record_id):
     buf = ctypes.create_string_buffer(1 << 16)
     check(lib().hmx_plan_text(int(record_id), buf, len(buf)), "plan")
+    return buf.value.decode()
+
+
+def program_source(record_id, compile=False):
+    """HIP source of the kernels generated for the record's fused groups;
+    with compile=True each is also built with hipRTC (raises on failure)."""
+    buf = ctypes.create_string_buffer(1 << 20)
+    check(lib().hmx_program_source(int(record_id), 1 if compile else 0, buf, len(buf)),
+          "program_source")
     return buf.value.decode()
 
 

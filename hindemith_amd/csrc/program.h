@@ -12,6 +12,9 @@
 
 #include "hmx_common.h"
 
+#include <string>
+#include <vector>
+
 namespace hmx {
 
 constexpr int PROG_MAX_INSTR = 64;
@@ -58,6 +61,29 @@ struct ScalarProg {
 };
 
 int launch_program(const ProgArgs &a, hipStream_t s, int *nblocks_out);
+
+// What a generated kernel is specialised on: the instruction list, the slot
+// counts, each scalar slot's dtype and each stencil / DIA operand's tap count
+// (fixed when the record is compiled; weights and offsets stay device data).
+struct ProgSpec {
+    std::vector<hmx_instr> code;
+    int H = 0;  // rows the launch covers (0: unknown)
+    int nin = 0, nout = 0, nscal = 0, nopnd = 0, nred = 0;
+    int scal_dt[PROG_MAX_SCAL] = {};
+    int opnd_nd[PROG_MAX_OPND] = {};
+};
+// Generated kernels (progjit.hip) unless HMX_PROGRAM_INTERP is set (then the
+// interpreter, program_kernel, runs every program).
+bool program_jit_enabled();
+// HIP source of the generated kernel for a program.
+int program_source(const ProgSpec &sp, std::string &src, std::string &err);
+// The compiled kernel for a program on a device (compiled with hipRTC on
+// first use, then cached for the life of the process).
+int program_jit_get(int device, const ProgSpec &sp, void **fn, std::string &err);
+// Compile generated source with hipRTC without loading it (no GPU needed).
+int program_jit_compile_only(const std::string &src, std::string &err);
+// Launch a generated kernel with the interpreter's arguments and grid.
+int launch_program_jit(void *fn, const ProgArgs &a, hipStream_t s);
 int program_grid_blocks(int H, int W);
 int launch_reduce_finalize(const double *part, int np, void *dst, int dt, hipStream_t s);
 int launch_scalar_program(const ScalarProg &p, hipStream_t s);

@@ -276,6 +276,7 @@ struct Group {
     int nreg = 0;
     int nblocks = 0;
     double *partials = nullptr;
+    void *jit = nullptr;  // generated kernel (progjit.hip), or null: the interpreter
 };
 
 struct Node {
@@ -1182,6 +1183,23 @@ static void release_record(Record &R) {
 // ---------------------------------------------------------------------------
 // Execution
 
+// What a group's generated kernel is specialised on.
+static ProgSpec group_spec(const Record &R, const Group &g) {
+    ProgSpec sp;
+    sp.code = g.code;
+    sp.H = g.H;
+    sp.nin = (int)g.in_syms.size();
+    sp.nout = (int)g.out_syms.size();
+    sp.nscal = (int)g.scal_syms.size();
+    sp.nopnd = (int)g.opnd_syms.size();
+    sp.nred = (int)g.red_syms.size();
+    for (int i = 0; i < sp.nscal && i < PROG_MAX_SCAL; ++i)
+        sp.scal_dt[i] = R.syms[g.scal_syms[i]].dtype;
+    for (int i = 0; i < sp.nopnd && i < PROG_MAX_OPND; ++i)
+        sp.opnd_nd[i] = (int)R.syms[g.opnd_syms[i]].ndiags;
+    return sp;
+}
+
 static int run_group(Record &R, Group &g) {
     ProgArgs a;
     std::memset(&a, 0, sizeof(a));
@@ -1208,7 +1226,7 @@ static int run_group(Record &R, Group &g) {
         a.opnd[i].kind = s.kind == HMX_KIND_SPMAT2D || s.kind == HMX_KIND_SPMAT;
     }
     for (size_t k = 0; k < g.red_syms.size(); ++k) a.red_part[k] = g.partials + k * g.nblocks;
-    int rc = launch_program(a, R.stream, nullptr);
+    int rc = g.jit ? launch_program_jit(g.jit, a, R.stream) : launch_program(a, R.stream, nullptr);
     if (rc) return rc;
     for (size_t k = 0; k < g.red_syms.size(); ++k) {
         Sym &s = R.syms[g.red_syms[k]];
@@ -1557,11 +1575,19 @@ extern "C" int hmx_compile(int rec, const char *symtab_xml, const char *sm_xml,
     HMX_HIP_TRY(hipStreamCreateWithFlags(&R->stream, hipStreamNonBlocking));
     mark_buffers(*R, R->plan);
     rc = allocate(*R);
+    const bool jit = program_jit_enabled();
     for (Group &g : R->groups) {
         if (rc) break;
         g.nblocks = program_grid_blocks(g.H, g.W);
         if (!g.red_syms.empty())
             rc = dalloc(*R, (void **)&g.partials, (int64_t)g.nblocks * 8 * g.red_syms.size());
+        if (!rc && jit) {
+            // each fused group becomes its own kernel, as the reference
+            // generates and builds one OpenCL kernel per group at xmlcompile
+            std::string err;
+            rc = program_jit_get(R->device, group_spec(*R, g), &g.jit, err);
+            if (rc) rc = fail(rc, "generated program: " + err);
+        }
     }
     // HS nodes need two ping-pong planes
     std::function<void(const std::vector<Node> &)> hs_need;
@@ -1712,6 +1738,25 @@ extern "C" int hmx_plan_text(int rec, char *buf, int64_t cap) {
     if (!buf || cap <= 0) return fail(HMX_ERR_INVALID, "null buffer");
     size_t n = std::min<size_t>(s.size(), (size_t)cap - 1);
     std::memcpy(buf, s.data(), n);
+    buf[n] = 0;
+    return HMX_OK;
+}
+
+extern "C" int hmx_program_source(int rec, int compile, char *buf, int64_t cap) {
+    Record *R = find_record(rec);
+    if (!R) return fail(HMX_ERR_NOT_FOUND, "no record " + std::to_string(rec));
+    if (!buf || cap <= 0) return fail(HMX_ERR_INVALID, "null buffer");
+    std::string all;
+    for (size_t i = 0; i < R->groups.size(); ++i) {
+        const ProgSpec sp = group_spec(*R, R->groups[i]);
+        std::string src, err;
+        int rc = program_source(sp, src, err);
+        if (!rc && compile) rc = program_jit_compile_only(src, err);
+        if (rc) return fail(rc, "group " + std::to_string(i) + ": " + err);
+        all += "// group " + std::to_string(i) + "\n" + src;
+    }
+    size_t n = std::min<size_t>(all.size(), (size_t)cap - 1);
+    std::memcpy(buf, all.data(), n);
     buf[n] = 0;
     return HMX_OK;
 }

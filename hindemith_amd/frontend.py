@@ -681,6 +681,9 @@ class Record:
     def plan(self):
         return backend.plan(self.id)
 
+    def program_source(self, compile=False):
+        return backend.program_source(self.id, compile)
+
 
 def mycompile(source, kwargs, defines, fn_key=None):
     """PyAST.mycompile (PyAST.py:149-196): compile once per kwargs signature."""

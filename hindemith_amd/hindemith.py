@@ -52,6 +52,11 @@ class hm_fn:
         """Launch plan of the most recent call (one line per kernel group)."""
         return self.last_record.plan() if self.last_record else ""
 
+    def program_source(self, compile=False):
+        """HIP source of the kernels generated for the most recent call's fused
+        groups (compile=True also builds them with hipRTC)."""
+        return self.last_record.program_source(compile) if self.last_record else ""
+
 
 def hm(fn):
     return hm_fn(fn)

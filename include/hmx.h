@@ -93,6 +93,11 @@ int hmx_result_info(int rec, int i, int *kind, int *dtype, int *ndim, int64_t *s
 int hmx_result_read(int rec, int i, void *host, int64_t nbytes);
 /* Human-readable plan: one line per launch group. */
 int hmx_plan_text(int rec, char *buf, int64_t cap);
+/* HIP source of the kernel generated for each fused group of the record (the
+ * reference's generated OpenCL, oclgen.cpp:95-154, which its compile log
+ * showed); compile != 0 also builds each with hipRTC for gfx950 (no GPU
+ * needed) and fails with the compiler's log if one does not build. */
+int hmx_program_source(int rec, int compile, char *buf, int64_t cap);
 /* Free a record's device buffers and graphs (free_ocl, backend.cpp:646-651,
  * was a no-op). */
 int hmx_release(int rec);

@@ -148,19 +148,42 @@ struct HsWindow {
 // fixed tap count and no LOUT.  On by default where it fits the registers of
 // two 1024-thread work-groups per CU: the double-buffered builds without
 // FDIV, whose second barrier's held results free the registers it needs.
+// COH: u, v are read and written with system-scope accesses (sc0 sc1), which
+// the hardware keeps coherent across the XCDs' L2 caches; for a launch whose
+// tiles read other tiles' output of the same launch (hs_persist_kernel).
+template <bool COH>
+__device__ __forceinline__ float hs_ld_uv(const char *p) {
+    if (COH)
+        return __hip_atomic_load(reinterpret_cast<const float *>(p), __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_SYSTEM);
+    return *reinterpret_cast<const float *>(p);
+}
+template <bool COH>
+__device__ __forceinline__ void hs_st_uv(char *p, float x) {
+    if (COH)
+        __hip_atomic_store(reinterpret_cast<float *>(p), x, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
+    else
+        *reinterpret_cast<float *>(p) = x;
+}
+
 template <int NT, int TW, int TH, int KMAX, int NTAP, bool LOUT = false, bool DB = false,
           bool FDIV = false,
           bool SHARED = !LOUT && ((TW + 2 * KMAX) * (TH + 2 * KMAX) + NT - 1) / NT <= 8,
-          bool PF = SHARED && DB>
+          bool PF = SHARED && DB, bool COH = false>
 __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
                                         const HsTapTable<TW + 2 * KMAX> &tp, float2 *uv,
-                                        float *lin = nullptr) {
+                                        float *lin = nullptr, int tid_in = -1) {
     static_assert(!(LOUT && DB), "the linearisation output needs the two-barrier step");
     constexpr bool PREFETCH = PF && SHARED && NTAP > 0 && !LOUT;
     constexpr int PITCH = TW + 2 * KMAX, RHM = TH + 2 * KMAX;
     constexpr int NPX = PITCH * RHM;
     constexpr int P = (NPX + NT - 1) / NT;
-    const int tid = threadIdx.x;
+    // tid_in: the thread index as a caller that loops over tiles passes it,
+    // made opaque each iteration so that nothing derived from it is hoisted
+    // out of the caller's loop (and kept live across tiles)
+    const int tid = tid_in >= 0 ? tid_in : (int)threadIdx.x;
+    __builtin_assume(tid >= 0 && tid < NT);
     const int tyi = t / a.tiles_x, txi = t - tyi * a.tiles_x;
     const int x0 = txi * TW, y0 = tyi * TH;
     const int x1 = min(x0 + TW, a.W), y1 = min(y0 + TH, a.H);
@@ -224,8 +247,8 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
                 cx[j] = *reinterpret_cast<const float *>(pIx + o);
                 cy[j] = *reinterpret_cast<const float *>(pIy + o);
                 ct[j] = *reinterpret_cast<const float *>(pIt + o);
-                ui[j] = *reinterpret_cast<const float *>(pu + o);
-                vi[j] = *reinterpret_cast<const float *>(pv + o);
+                ui[j] = hs_ld_uv<COH>(pu + o);
+                vi[j] = hs_ld_uv<COH>(pv + o);
                 if (pd) cd[j] = *reinterpret_cast<const float *>(pd + o);
             }
         }
@@ -282,8 +305,8 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
             const unsigned o = (unsigned)clampi(gy, 0, a.H - 1) * rowb +
                                (unsigned)clampi(gx, 0, a.W - 1) * (unsigned)sizeof(float);
             if (in[j]) {
-                ui[j] = *reinterpret_cast<const float *>(bu + o);
-                vi[j] = *reinterpret_cast<const float *>(bv + o);
+                ui[j] = hs_ld_uv<COH>(bu + o);
+                vi[j] = hs_ld_uv<COH>(bv + o);
             }
             if (img[j]) {
                 cx[j] = *reinterpret_cast<const float *>(bIx + o);
@@ -485,8 +508,8 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
 #pragma unroll
         for (int j = 0; j < NO; ++j) {
             if (st[j]) {
-                *reinterpret_cast<float *>(pu + o[j]) = q[j].x;
-                *reinterpret_cast<float *>(pv + o[j]) = q[j].y;
+                hs_st_uv<COH>(pu + o[j], q[j].x);
+                hs_st_uv<COH>(pv + o[j], q[j].y);
             }
         }
     }

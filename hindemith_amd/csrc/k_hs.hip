@@ -23,6 +23,7 @@
 #include "hs_tile.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <utility>
@@ -125,6 +126,112 @@ __global__ __launch_bounds__(NT) void hs_strip_kernel(HsArgs a, Taps taps) {
 }
 
 // ---------------------------------------------------------------------------
+// Persistent form (variant 28): one launch runs every block of k iterations.
+// The grid is resident (two 1024-thread work-groups per CU); XCD x (work-group
+// index % 8) owns the band [x*T/8, (x+1)*T/8) of the row-major tile order and
+// its work-groups take the band's tiles in turn.  cnt[t] counts the blocks
+// tile t has completed; block b of a tile starts once its eight neighbours have
+// completed block b-1, which also means they have finished reading the planes
+// block b overwrites (block b reads plane (b-1)&1 and writes plane b&1; block
+// 0 reads the caller's u, v and the last block writes them).  Every access to
+// the planes and the counters is a system-scope access (sc0 sc1), which the
+// hardware keeps coherent across the XCDs' L2 caches, so a tile's output is
+// visible to every XCD once its stores have completed and no L2 write-back
+// or invalidation is needed (with plain accesses, even to the uncached
+// allocation the planes live in, a neighbour on another XCD read stale halo
+// values).  Waits are bounded; a work-group that gives up sets *err and every
+// wait then ends.
+struct PersistArgs {
+    const float *Ix, *Iy, *It, *den;
+    const float *u0, *v0;  // the caller's planes: read by block 0 ...
+    float *uN, *vN;        // ... and written by the last block
+    float *pu[2], *pv[2];  // ping-pong planes (uncached)
+    int *cnt;              // per tile: blocks completed (uncached)
+    int *err;              // nonzero once a wait has given up (uncached)
+    float alpha2;
+    int H, W, tiles_x, tiles_y, ntiles;
+    int nblocks, k, k_last;
+    int spin_limit;
+};
+
+constexpr int kPersistXcds = 8;
+
+template <int NTAP>
+__global__ __launch_bounds__(1024, 8) void hs_persist_kernel(PersistArgs p, Taps taps) {
+    using Win = HsWindow<1024, 64, 32, 8, true>;
+    __shared__ float2 uvm[Win::SIZE];
+    __shared__ int s_fail;
+    const HsTapTable<64 + 2 * 8> tp(taps, NTAP);
+    HsImage im;
+    im.Ix = p.Ix;
+    im.Iy = p.Iy;
+    im.It = p.It;
+    im.den = p.den;
+    im.alpha2 = p.alpha2;
+    im.H = p.H;
+    im.W = p.W;
+    im.tiles_x = p.tiles_x;
+    const int tid = threadIdx.x;
+    const int xcd = blockIdx.x % kPersistXcds, slot = blockIdx.x / kPersistXcds;
+    const int per = gridDim.x / kPersistXcds;  // the host launches a multiple of 8
+    const int band0 = (int)((int64_t)p.ntiles * xcd / kPersistXcds);
+    const int band1 = (int)((int64_t)p.ntiles * (xcd + 1) / kPersistXcds);
+    for (int b = 0; b < p.nblocks; ++b) {
+        const bool last = b == p.nblocks - 1;
+        im.u_in = b == 0 ? p.u0 : p.pu[(b - 1) & 1];
+        im.v_in = b == 0 ? p.v0 : p.pv[(b - 1) & 1];
+        im.u_out = last ? p.uN : p.pu[b & 1];
+        im.v_out = last ? p.vN : p.pv[b & 1];
+        const int kk = last ? p.k_last : p.k;
+        for (int t = band0 + slot; t < band1; t += per) {
+            if (b > 0) {
+                if (tid == 0) s_fail = 0;
+                __syncthreads();
+                if (tid < 9 && tid != 4) {  // one lane per neighbour
+                    const int ty = t / p.tiles_x, tx = t - ty * p.tiles_x;
+                    const int nx = tx + tid % 3 - 1, ny = ty + tid / 3 - 1;
+                    if (nx >= 0 && nx < p.tiles_x && ny >= 0 && ny < p.tiles_y) {
+                        const int *c = p.cnt + ny * p.tiles_x + nx;
+                        int spins = 0;
+                        while (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) <
+                               b) {
+                            if (++spins > p.spin_limit ||
+                                __hip_atomic_load(p.err, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_SYSTEM) != 0) {
+                                atomicOr(&s_fail, 1);
+                                break;
+                            }
+                            __builtin_amdgcn_s_sleep(2);
+                        }
+                    }
+                }
+                __syncthreads();
+                if (s_fail) {  // uniform: every thread leaves
+                    if (tid == 0) __hip_atomic_store(p.err, 1, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_SYSTEM);
+                    return;
+                }
+            }
+            int otid = tid;  // opaque per tile: see hs_tile's tid_in
+            asm volatile("" : "+v"(otid));
+            hs_tile<1024, 64, 32, 8, NTAP, false, true, true, true, true, true>(
+                im, t, kk, tp, uvm + Win::PAD, nullptr, otid);
+            // every wave's stores of the tile have completed (in memory: the
+            // planes are uncached) before the tile's count moves on
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            if (tid == 0)
+                __hip_atomic_store(p.cnt + t, b + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+    }
+}
+
+__global__ void hs_persist_reset_kernel(int *cnt, int n, int *err) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) cnt[i] = 0;
+    if (i == 0) *err = 0;
+}
+
 // Variant table.  (threads, tile, max iterations per launch); LDS use is
 // 8*(TW+2K)(TH+2K) bytes.
 struct HsVariant {
@@ -132,6 +239,7 @@ struct HsVariant {
     int strip;                        // rows per thread per column (0: rectangular tile)
     void (*kernel[5])(HsArgs, Taps);  // by tap class: 4, 5, 8, 9 taps, any
     void (*ring)(HsArgs, Taps);       // ring-ordered kernel for the canonical 8 taps, or null
+    bool persist;                     // whole solve in one persistent launch (hs_persist_run)
 };
 
 // tap count -> kernel slot (straight-line kernels for the common stencils)
@@ -207,6 +315,16 @@ static const HsVariant kVariants[] = {
       hs_tb_kernel<1024, 64, 32, 8, 9, false, true, true>,
       hs_tb_kernel<1024, 64, 32, 8, 0, false, true, true>},
      hs_ring_kernel<true>},
+    // 28: as 24, every block of k iterations in one persistent launch
+    // (hs_persist_kernel); a solve of one block, or a batch of images, runs
+    // variant 24's launches
+    {1024, 64, 32, 8, 0,
+     {hs_tb_kernel<1024, 64, 32, 8, 4, false, true, true>,
+      hs_tb_kernel<1024, 64, 32, 8, 5, false, true, true>,
+      hs_tb_kernel<1024, 64, 32, 8, 8, false, true, true>,
+      hs_tb_kernel<1024, 64, 32, 8, 9, false, true, true>,
+      hs_tb_kernel<1024, 64, 32, 8, 0, false, true, true>},
+     nullptr, true},
 };
 #undef HS_V
 #undef HS_D
@@ -312,11 +430,141 @@ int hs_launch_count(int iters, int k) { return iters <= 0 ? 0 : (iters + k - 1) 
 // start state is first copied into (wu,wv) (one pass over both planes) and
 // the first launch reads it from there; start_in_work: the caller has already
 // put the start state there (hmx_hs_solve's setup does it in the same pass).
+// Uncached ping-pong planes and tile counters of the persistent form, one set
+// per device, grown on demand.  Uncached memory is never freed: a buffer
+// freed after a kernel has read it has corrupted later allocations on this
+// platform, so a larger image leaves the old buffer allocated.  Persistent
+// solves that overlap on one device share these buffers and race.
+struct PersistBuf {
+    float *planes = nullptr;  // 4 planes of cap_px floats
+    int *cnt = nullptr;       // cap_words ints: tile counters, then the error word
+    int64_t cap_px = 0;
+    int cap_words = 0;
+};
+static std::mutex g_persist_mu;
+static std::map<int, PersistBuf> g_persist;
+
+static int persist_buffers(int device, int64_t px, int words, PersistBuf &out) {
+    std::lock_guard<std::mutex> lk(g_persist_mu);
+    PersistBuf &P = g_persist[device];
+    if (P.cap_px < px) {
+        void *q = nullptr;
+        HMX_HIP_TRY(hipExtMallocWithFlags(&q, (size_t)px * 4 * sizeof(float),
+                                          hipDeviceMallocUncached));
+        P.planes = static_cast<float *>(q);
+        P.cap_px = px;
+    }
+    if (P.cap_words < words) {
+        void *q = nullptr;
+        HMX_HIP_TRY(hipExtMallocWithFlags(&q, (size_t)words * sizeof(int), hipDeviceMallocUncached));
+        P.cnt = static_cast<int *>(q);
+        P.cap_words = words;
+    }
+    out = P;
+    return HMX_OK;
+}
+
+// Work-groups of the persistent kernel that are resident together: the
+// runtime's figure, bounded by the waves per SIMD the kernel is compiled for
+// (two work-groups per CU; the runtime's figure does not see SGPRs), its
+// VGPRs and LDS; a multiple of the XCD count.
+static int persist_grid(const void *kern, int device, int ntiles, int &grid) {
+    int api = 0, cus = 0, lds = 0;
+    HMX_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&api, kern, 1024, 0));
+    HMX_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    HMX_HIP_TRY(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                                      device));
+    hipFuncAttributes fa;
+    HMX_HIP_TRY(hipFuncGetAttributes(&fa, kern));
+    const int vgpr = std::max(8, (fa.numRegs + 7) / 8 * 8);
+    int own = std::min(8, 512 / vgpr) / 4;  // 16 waves per work-group = 4 per SIMD
+    if (fa.sharedSizeBytes > 0 && lds > 0) own = std::min(own, (int)(lds / fa.sharedSizeBytes));
+    // (a few bytes of private memory per lane -- the 8- and 9-tap builds spill
+    // one or a few values per tile -- do not limit the waves a CU holds; a
+    // grid that still is not resident ends in the bounded waits' error)
+    const int cap = std::min(api, own) * cus / kPersistXcds * kPersistXcds;
+    if (cap < kPersistXcds) return fail(HMX_ERR_UNSUPPORTED, "persistent HS kernel cannot be resident");
+    grid = std::min(cap, (ntiles + kPersistXcds - 1) / kPersistXcds * kPersistXcds);
+    return HMX_OK;
+}
+
+// Whether hs_jacobi_run takes the persistent path for this call.
+static bool hs_runs_persistent(int variant, int iters, int k, int B, const float *alpha2_b) {
+    return kVariants[variant].persist && B == 1 && !alpha2_b && k >= 1 && k <= 8 &&
+           hs_launch_count(iters, k) >= 2;
+}
+
+static int hs_persist_run(const float *Ix, const float *Iy, const float *It, const float *den,
+                          float alpha2, int H, int W, const Taps &taps, int iters, float *u,
+                          float *v, int k, hipStream_t s) {
+    static void (*const kPersist[5])(PersistArgs, Taps) = {
+        hs_persist_kernel<4>, hs_persist_kernel<5>, hs_persist_kernel<8>, hs_persist_kernel<9>,
+        hs_persist_kernel<0>};
+    auto kern = kPersist[tap_class(taps.nd)];
+    int device = 0;
+    HMX_HIP_TRY(hipGetDevice(&device));
+    PersistArgs p;
+    p.Ix = Ix;
+    p.Iy = Iy;
+    p.It = It;
+    p.den = den;
+    p.alpha2 = alpha2;
+    p.H = H;
+    p.W = W;
+    p.tiles_x = (W + 63) / 64;
+    p.tiles_y = (H + 31) / 32;
+    p.ntiles = p.tiles_x * p.tiles_y;
+    p.nblocks = hs_launch_count(iters, k);
+    p.k = k;
+    p.k_last = iters - k * (p.nblocks - 1);
+    // polls per wait before giving up (about a second); HMX_PERSIST_SPIN_LIMIT
+    // lowers it so a test can make the waits give up
+    p.spin_limit = 1 << 20;
+    if (const char *e = std::getenv("HMX_PERSIST_SPIN_LIMIT")) p.spin_limit = std::max(1, std::atoi(e));
+    int grid = 0;
+    int rc = persist_grid(reinterpret_cast<const void *>(kern), device, p.ntiles, grid);
+    if (rc) return rc;
+    const int64_t px = (int64_t)H * W;
+    PersistBuf buf;
+    rc = persist_buffers(device, px, p.ntiles + 1, buf);
+    if (rc) return rc;
+    for (int i = 0; i < 2; ++i) {
+        p.pu[i] = buf.planes + (2 * i) * px;
+        p.pv[i] = buf.planes + (2 * i + 1) * px;
+    }
+    p.cnt = buf.cnt;
+    p.err = buf.cnt + p.ntiles;
+    p.u0 = u;
+    p.v0 = v;
+    p.uN = u;
+    p.vN = v;
+    // counters and the error word start at zero (a fill kernel: the call may
+    // be captured in a graph)
+    hipLaunchKernelGGL(hs_persist_reset_kernel, dim3((p.ntiles + 255) / 256), dim3(256), 0, s,
+                       p.cnt, p.ntiles, p.err);
+    HMX_LAUNCH_CHECK("hs_persist_reset_kernel");
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(1024), 0, s, p, taps);
+    HMX_LAUNCH_CHECK("hs_persist_kernel");
+    // outside graph capture, read back whether a wait gave up (a grid that
+    // was not resident); a captured call cannot synchronise
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HMX_HIP_TRY(hipStreamIsCapturing(s, &cs));
+    if (cs == hipStreamCaptureStatusNone) {
+        int h = 0;
+        HMX_HIP_TRY(hipMemcpyAsync(&h, p.err, sizeof(int), hipMemcpyDeviceToHost, s));
+        HMX_HIP_TRY(hipStreamSynchronize(s));
+        if (h) return fail(HMX_ERR_TIMEOUT, "persistent HS solve: a tile's wait gave up");
+    }
+    return HMX_OK;
+}
+
 int hs_jacobi_run(const float *Ix, const float *Iy, const float *It, const float *den,
                   float alpha2, int H, int W, const Taps &taps, int iters, float *u, float *v,
                   float *wu, float *wv, int variant, int k, hipStream_t s, int B,
                   const float *alpha2_b, bool start_in_work) {
     if (iters <= 0) return HMX_OK;
+    if (hs_runs_persistent(variant, iters, k, B, alpha2_b))
+        return hs_persist_run(Ix, Iy, It, den, alpha2, H, W, taps, iters, u, v, k, s);
     const bool odd = hs_launch_count(iters, k) & 1;
     float *src_u = u, *src_v = v, *dst_u = wu, *dst_v = wv;
     if (odd) {
@@ -507,6 +755,10 @@ extern "C" int hmx_hs_tune(const float *Ix, const float *Iy, const float *It, fl
         for (int k = 1; k <= kVariants[vi].kmax && !rc; k *= 2) {
             float ms;
             rc = time_cfg(vi, k, 48, 2, ms);
+            if (rc == HMX_ERR_TIMEOUT) {  // a persistent grid that gave up: not a candidate
+                rc = HMX_OK;
+                continue;
+            }
             cands.push_back({ms, vi, k});
         }
     int best_v = 0, best_k = 1;
@@ -519,6 +771,10 @@ extern "C" int hmx_hs_tune(const float *Ix, const float *Iy, const float *It, fl
         for (size_t i = 0; i < cands.size() && i < 4 && !rc; ++i) {
             float ms;
             rc = time_cfg(cands[i].v, cands[i].k, 240, 3, ms);
+            if (rc == HMX_ERR_TIMEOUT) {
+                rc = HMX_OK;
+                continue;
+            }
             if (!rc && cands[i].v == dv && cands[i].k == dk) dms = ms;
             if (!rc && ms < best) {
                 best = ms;
@@ -575,8 +831,9 @@ extern "C" int hmx_hs_solve(const float *I1, const float *I2, int H, int W, floa
     int variant, k;
     hs_choice(H, W, variant, k);
     // an odd launch count starts from the work planes: the setup pass copies
-    // (u, v) there as it goes
-    const bool odd = hs_launch_count(iters, k) & 1;
+    // (u, v) there as it goes (the persistent form reads u, v itself)
+    const bool odd = !hs_runs_persistent(variant, iters, k, 1, nullptr) &&
+                     (hs_launch_count(iters, k) & 1);
     rc = launch_hs_setup(I1, I2, H, W, dx, dy, dt, Ix, Iy, It, s, odd ? u : nullptr,
                          odd ? v : nullptr, wu, wv);
     if (rc) return rc;

@@ -88,11 +88,11 @@ def test_hs_solve_default_config(K, shape):
 
 
 # One variant per tile-body form: 24 the default (double-buffered, reciprocal
-# quotient), 27 its ring-ordered form, 17 double-buffered with the IEEE
-# division, 3 the single-window two-barrier step, 12 the column-strip body, 0
-# one plain sweep per launch.
-@pytest.mark.parametrize("variant,k", [(24, 8), (24, 7), (27, 8), (27, 5), (17, 8), (3, 8),
-                                       (12, 8), (0, 1)])
+# quotient), 27 its ring-ordered form, 28 the persistent launch of 24's tile
+# body, 17 double-buffered with the IEEE division, 3 the single-window
+# two-barrier step, 12 the column-strip body, 0 one plain sweep per launch.
+@pytest.mark.parametrize("variant,k", [(24, 8), (24, 7), (27, 8), (27, 5), (28, 8), (28, 5),
+                                       (17, 8), (3, 8), (12, 8), (0, 1)])
 def test_hs_jacobi_1080p_variants(K, variant, k):
     c = case(1080, 1920)
     I1, I2 = torch.from_numpy(c.I1).cuda(), torch.from_numpy(c.I2).cuda()

@@ -4,11 +4,14 @@ Bit-exact where the kernel performs the reference's operations in the same
 order (stencil, DIA, single-lane CSR, HS setup, every HS Jacobi variant);
 within stated bounds where the reduction order differs (multi-lane CSR, sum2d).
 """
+import os
+
 import numpy as np
 import pytest
 
 import oracle
 from hindemith_amd import synthetic as S
+from hindemith_amd._lib import HindemithError
 
 torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
@@ -352,6 +355,52 @@ def test_device_reciprocal_exhaustive(K):
     assert K.hs_recip_check(-24, 24) == 0
     assert K.hs_recip_check(-40, -25) == 0
     assert K.hs_recip_check(25, 40) == 0
+
+
+def test_persistent_solve_many_blocks_and_sizes(K):
+    """Variant 28 runs every block of k iterations in one persistent launch,
+    tiles waiting on their neighbours' counters: bit-exact with the oracle for
+    images with fewer tiles than work-groups, one tile row or column, and more
+    tiles than the resident grid; every k; a second call on the same buffers
+    (counters reset) and a larger image after a smaller one (buffers grown)."""
+    for (H, W), iters in [((33, 70), 40), ((1, 513), 17), ((700, 1), 9), ((300, 1100), 30),
+                          ((1200, 2100), 24), ((64, 64), 16)]:
+        I1, I2 = S.hs_pair(H, W, seed=H + W)[:2]
+        Ix, Iy, It = oracle.hs_setup(I1, I2, TAPS["Dx"], TAPS["Dy"], TAPS["Dt"])
+        ref_u, ref_v = oracle.hs_jacobi(Ix, Iy, It, 1.0, TAPS["Avg"], iters)
+        dIx, dIy, dIt = (torch.from_numpy(a).cuda() for a in (Ix, Iy, It))
+        for k in (1, 3, 8):
+            for rep in range(2):
+                u = torch.zeros((H, W), device="cuda")
+                v = torch.zeros_like(u)
+                K.hs_jacobi(dIx, dIy, dIt, 1.0, TAPS["Avg"], iters, u, v, variant=28, k=k)
+                torch.cuda.synchronize()
+                assert np.array_equal(u.cpu().numpy(), ref_u), (H, W, iters, k, rep)
+                assert np.array_equal(v.cpu().numpy(), ref_v), (H, W, iters, k, rep)
+
+
+def test_persistent_solve_forced_timeout_reports_error(K):
+    """A spin limit of one poll makes the neighbour waits give up (forced from
+    the host: every work-group still exits); the call reports a time-out, and
+    the next call with the normal limit is bit-exact again."""
+    H, W, iters = 256, 512, 40
+    I1, I2 = S.hs_pair(H, W, seed=3)[:2]
+    Ix, Iy, It = oracle.hs_setup(I1, I2, TAPS["Dx"], TAPS["Dy"], TAPS["Dt"])
+    dIx, dIy, dIt = (torch.from_numpy(a).cuda() for a in (Ix, Iy, It))
+    u = torch.zeros((H, W), device="cuda")
+    v = torch.zeros_like(u)
+    os.environ["HMX_PERSIST_SPIN_LIMIT"] = "1"
+    try:
+        with pytest.raises(HindemithError) as ei:
+            K.hs_jacobi(dIx, dIy, dIt, 1.0, TAPS["Avg"], iters, u, v, variant=28, k=1)
+        assert ei.value.code == 6  # HMX_ERR_TIMEOUT
+    finally:
+        del os.environ["HMX_PERSIST_SPIN_LIMIT"]
+    u.zero_()
+    v.zero_()
+    K.hs_jacobi(dIx, dIy, dIt, 1.0, TAPS["Avg"], iters, u, v, variant=28, k=8)
+    ref_u, ref_v = oracle.hs_jacobi(Ix, Iy, It, 1.0, TAPS["Avg"], iters)
+    assert np.array_equal(u.cpu().numpy(), ref_u) and np.array_equal(v.cpu().numpy(), ref_v)
 
 
 def test_hs_jacobi_division_edge_cases(K):

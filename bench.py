@@ -224,6 +224,11 @@ def extra_bandwidth(K, S):
     ms = _time_graph_ms(lambda: K.dia_spmv(D1, off1, x.reshape(-1), out=y.reshape(-1)))
     out["dia1d_5diag"] = {"us": round(ms * 1e3, 2), "GBps": round(28 * N / ms / 1e6, 1)}
     del D1
+    # sum2d: deterministic two-pass reduction, 4 B per element read
+    part = torch.empty(1024, device="cuda", dtype=torch.float64)
+    s1 = torch.empty(1, device="cuda")
+    ms = _time_graph_ms(lambda: K.sum2d(x, partials=part, out=s1))
+    out["sum2d"] = {"us": round(ms * 1e3, 2), "GBps": round(4 * N / ms / 1e6, 1)}
     ip, ix, d = S.dia2d_to_csr(planes, ox, oy)
     tip, tix, td = (torch.from_numpy(a).cuda() for a in (ip, ix, d))
     xf = x.reshape(-1)

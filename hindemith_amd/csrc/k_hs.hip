@@ -378,10 +378,14 @@ int launch_hs_tb(int variant, const float *Ix, const float *Iy, const float *It,
     if (V.ring && hs_ring_taps(taps)) kern = V.ring;
     if (lt || lub) {
         // linearisation output exists only for the 1024-thread 64x32 k<=8 shape
+        // (with the reciprocal quotient: the same bits as the division, fewer
+        // VALU per step; 59 VGPRs at most, within the 8 waves per SIMD)
         static void (*const kLinKernels[5])(HsArgs, Taps) = {
-            hs_tb_kernel<1024, 64, 32, 8, 4, true>, hs_tb_kernel<1024, 64, 32, 8, 5, true>,
-            hs_tb_kernel<1024, 64, 32, 8, 8, true>, hs_tb_kernel<1024, 64, 32, 8, 9, true>,
-            hs_tb_kernel<1024, 64, 32, 8, 0, true>};
+            hs_tb_kernel<1024, 64, 32, 8, 4, true, false, true>,
+            hs_tb_kernel<1024, 64, 32, 8, 5, true, false, true>,
+            hs_tb_kernel<1024, 64, 32, 8, 8, true, false, true>,
+            hs_tb_kernel<1024, 64, 32, 8, 9, true, false, true>,
+            hs_tb_kernel<1024, 64, 32, 8, 0, true, false, true>};
         if (V.strip || V.nt != 1024 || V.tw != 64 || V.th != 32 || V.kmax != 8)
             return fail(HMX_ERR_UNSUPPORTED,
                         "hs_tb: linearisation output needs the 1024x64x32x8 variant");

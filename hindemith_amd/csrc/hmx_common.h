@@ -57,6 +57,15 @@ __host__ __device__ inline int clampi(int v, int lo, int hi) {
     return v < lo ? lo : (v > hi ? hi : v);
 }
 
+// A kernel's output store that writes through the L2 (a system-scope store,
+// sc0 sc1).  The next kernel reads the output from memory anyway (a kernel
+// boundary invalidates the L2); written through, it leaves no dirty lines for
+// the end of the launch to write back, which otherwise follows the last
+// tile's work instead of overlapping it.
+__device__ __forceinline__ void st_out(float *p, float x) {
+    __hip_atomic_store(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // blockIdx -> logical tile index so that the tiles one XCD receives (blocks
 // are dealt round-robin over the 8 XCDs) are spatially contiguous, which lets
 // neighbouring tiles share halo lines in that XCD's L2.

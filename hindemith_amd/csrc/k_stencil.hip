@@ -89,7 +89,7 @@ __global__ __launch_bounds__(256) void stencil2d_cols_kernel(const float *__rest
     }
 #pragma unroll
     for (int v = 0; v < ST_V; ++v)
-        if (r0 + v < H) y[(int64_t)(r0 + v) * W + c] = acc[v];
+        if (r0 + v < H) st_out(y + (int64_t)(r0 + v) * W + c, acc[v]);
 }
 
 // Wide stencils (radius > 4): taps straight from global memory (L1/L2 serve
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(256) void dia2d_kernel(const float *__restrict__ A,
                 }
             }
         }
-        y[p] = acc;
+        st_out(y + p, acc);
     }
 }
 

@@ -57,14 +57,21 @@ __host__ __device__ inline int clampi(int v, int lo, int hi) {
     return v < lo ? lo : (v > hi ? hi : v);
 }
 
-// A kernel's output store that writes through the L2 (a system-scope store,
-// sc0 sc1).  The next kernel reads the output from memory anyway (a kernel
-// boundary invalidates the L2); written through, it leaves no dirty lines for
-// the end of the launch to write back, which otherwise follows the last
-// tile's work instead of overlapping it.
-__device__ __forceinline__ void st_out(float *p, float x) {
-    __hip_atomic_store(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+// A kernel's output store, written through the L2 (a system-scope store,
+// sc0 sc1) when wt.  The next kernel reads the output from memory anyway (a
+// kernel boundary invalidates the L2); written through, it leaves no dirty
+// lines for the end of the launch to write back, which otherwise follows the
+// last work-group's work.  Only whole cache lines gain: a partly written line
+// becomes a partial write to memory that the L2 would have merged, so kernels
+// pass wt = rows_whole_lines(W), true when every image row starts on a
+// 128-byte line (a wave's 64 consecutive floats then fill whole lines).
+__device__ __forceinline__ void st_out(float *p, float x, bool wt) {
+    if (wt)
+        __hip_atomic_store(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    else
+        *p = x;
 }
+__host__ __device__ inline bool rows_whole_lines(int W) { return (W & 31) == 0; }
 
 // blockIdx -> logical tile index so that the tiles one XCD receives (blocks
 // are dealt round-robin over the 8 XCDs) are spatially contiguous, which lets

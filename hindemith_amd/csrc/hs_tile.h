@@ -158,16 +158,13 @@ __device__ __forceinline__ float hs_ld_uv(const char *p) {
                                  __HIP_MEMORY_SCOPE_SYSTEM);
     return *reinterpret_cast<const float *>(p);
 }
-// The output tile is always stored with system-scope stores (sc0 sc1), which
-// write through the L2: the next launch reads it from memory anyway (a kernel
-// boundary invalidates the L2), and a written-through tile leaves no dirty
-// lines for the end of the launch to write back (1080p launches 31.9 ->
-// 31.3 us on one box; non-temporal stores were slower, 32.4 us).
+// The output tile is stored through the L2 (st_out) when the image's rows
+// fill whole cache lines, as every kernel's output here (1080p launches 31.9
+// -> 31.3 us on one box; non-temporal stores were slower, 32.4 us); COH
+// stores always use system scope.
 template <bool COH>
-__device__ __forceinline__ void hs_st_uv(char *p, float x) {
-    (void)COH;
-    __hip_atomic_store(reinterpret_cast<float *>(p), x, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_SYSTEM);
+__device__ __forceinline__ void hs_st_uv(char *p, float x, bool wt) {
+    st_out(reinterpret_cast<float *>(p), x, COH || wt);
 }
 
 template <int NT, int TW, int TH, int KMAX, int NTAP, bool LOUT = false, bool DB = false,
@@ -460,9 +457,9 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
                 const int gy = y0 + i / TW, gx = x0 + i % TW;
                 if (gx < x1 && gy < y1) {
                     const int64_t g = so + (int64_t)gy * a.W + gx;
-                    if (a.lt) a.lt[g] = lin[i];  // (t is optional)
-                    a.lub[g] = lin[TW * TH + i];
-                    a.lvb[g] = lin[2 * TW * TH + i];
+                    if (a.lt) st_out(a.lt + g, lin[i], rows_whole_lines(a.W));  // (t is optional)
+                    st_out(a.lub + g, lin[TW * TH + i], rows_whole_lines(a.W));
+                    st_out(a.lvb + g, lin[2 * TW * TH + i], rows_whole_lines(a.W));
                 }
             }
         }
@@ -494,6 +491,7 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
         char *const pv = reinterpret_cast<char *>(a.v_out) + corner;
         const unsigned rowb = (unsigned)a.W * (unsigned)sizeof(float);
         const int tw = x1 - x0, th = y1 - y0;
+        const bool wt = rows_whole_lines(a.W);
         constexpr int NO = (TW * TH + NT - 1) / NT;
         // all of a thread's cells are read before the first store, so no
         // store waits for an earlier one to leave the registers it reads
@@ -511,8 +509,8 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
 #pragma unroll
         for (int j = 0; j < NO; ++j) {
             if (st[j]) {
-                hs_st_uv<COH>(pu + o[j], q[j].x);
-                hs_st_uv<COH>(pv + o[j], q[j].y);
+                hs_st_uv<COH>(pu + o[j], q[j].x, wt);
+                hs_st_uv<COH>(pv + o[j], q[j].y, wt);
             }
         }
     }

@@ -378,12 +378,12 @@ __global__ __launch_bounds__(NT, 8) void hs_adj_tb_kernel(AdjTbArgs a, Taps taps
         if (pk[j] < 0) continue;
         const int rx = pk[j] & 0xfff, ry = (pk[j] >> 16) & 0x7ff;
         const int64_t q = off + (int64_t)(wy0 + ry) * W + wx0 + rx;
-        a.gu_out[q] = g[j].x;
-        a.gv_out[q] = g[j].y;
-        a.gIx[q] = ax[j];
-        a.gIy[q] = ay[j];
-        a.gIt[q] = at[j];
-        a.ga2[q] = aa[j];
+        st_out(a.gu_out + q, g[j].x, rows_whole_lines(a.W));
+        st_out(a.gv_out + q, g[j].y, rows_whole_lines(a.W));
+        st_out(a.gIx + q, ax[j], rows_whole_lines(a.W));
+        st_out(a.gIy + q, ay[j], rows_whole_lines(a.W));
+        st_out(a.gIt + q, at[j], rows_whole_lines(a.W));
+        st_out(a.ga2 + q, aa[j], rows_whole_lines(a.W));
     }
 }
 

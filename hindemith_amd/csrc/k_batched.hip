@@ -139,8 +139,8 @@ __device__ __forceinline__ void setup_tile(const BatchArgs &b, const SetupTaps &
         if (r >= P.H || c >= P.W) continue;
         const int64_t p = (int64_t)r * P.W + c;
         if (b.phase0) {  // the first phase reads the start state from (wu, wv)
-            P.wu[p] = P.u[p];
-            P.wv[p] = P.v[p];
+            st_out(P.wu + p, P.u[p], rows_whole_lines(P.W));
+            st_out(P.wv + p, P.v[p], rows_whole_lines(P.W));
         }
         if (TAP4) {
             const float a1 = setup_tap4(P.I1, P.H, P.W, r, c, st.dx);
@@ -149,20 +149,20 @@ __device__ __forceinline__ void setup_tile(const BatchArgs &b, const SetupTaps &
             const float b2 = setup_tap4(P.I2, P.H, P.W, r, c, st.dy);
             const float t2 = setup_tap4(P.I2, P.H, P.W, r, c, st.dt);
             const float t1 = setup_tap4(P.I1, P.H, P.W, r, c, st.dt);
-            P.Ix[p] = a1 + a2;
-            P.Iy[p] = b1 + b2;
-            P.It[p] = t2 - t1;
+            st_out(P.Ix + p, a1 + a2, rows_whole_lines(P.W));
+            st_out(P.Iy + p, b1 + b2, rows_whole_lines(P.W));
+            st_out(P.It + p, t2 - t1, rows_whole_lines(P.W));
             continue;
         }
         const float a1 = setup_tap(P.I1, P.H, P.W, r, c, st.dx);
         const float a2 = setup_tap(P.I2, P.H, P.W, r, c, st.dx);
-        P.Ix[p] = a1 + a2;
+        st_out(P.Ix + p, a1 + a2, rows_whole_lines(P.W));
         const float b1 = setup_tap(P.I1, P.H, P.W, r, c, st.dy);
         const float b2 = setup_tap(P.I2, P.H, P.W, r, c, st.dy);
-        P.Iy[p] = b1 + b2;
+        st_out(P.Iy + p, b1 + b2, rows_whole_lines(P.W));
         const float t2 = setup_tap(P.I2, P.H, P.W, r, c, st.dt);
         const float t1 = setup_tap(P.I1, P.H, P.W, r, c, st.dt);
-        P.It[p] = t2 - t1;
+        st_out(P.It + p, t2 - t1, rows_whole_lines(P.W));
     }
 }
 

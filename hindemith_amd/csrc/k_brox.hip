@@ -51,24 +51,24 @@ __global__ __launch_bounds__(256) void brox_set_kernel(float *__restrict__ A,
         const float ps = psi[p];
         if (r > 0) {
             const float pu = psi[p - W];
-            A[pt * HW + p - W] = nalpha * pu;
+            st_out(A + (pt * HW + p - W), nalpha * pu, rows_whole_lines(W));
             mid = fmaf(alpha, pu, mid);
         }
         if (c > 0) {
             const float pl_ = psi[p - 1];
-            A[pl * HW + p - 1] = nalpha * pl_;
+            st_out(A + (pl * HW + p - 1), nalpha * pl_, rows_whole_lines(W));
             mid = fmaf(alpha, pl_, mid);
         }
         if (c < W - 1) {
-            A[pr * HW + p + 1] = nalpha * ps;
+            st_out(A + (pr * HW + p + 1), nalpha * ps, rows_whole_lines(W));
             mid = fmaf(alpha, ps, mid);
         }
         if (r < H - 1) {
-            A[pb * HW + p + W] = nalpha * ps;
+            st_out(A + (pb * HW + p + W), nalpha * ps, rows_whole_lines(W));
             mid = fmaf(alpha, ps, mid);
         }
-        if (rb) diag[p] = mid;
-        else A[2 * HW + p] = mid;
+        if (rb) st_out(diag + p, mid, rows_whole_lines(W));
+        else st_out(A + (2 * HW + p), mid, rows_whole_lines(W));
     }
 }
 
@@ -92,11 +92,11 @@ __global__ __launch_bounds__(256) void brox_precond_kernel(
             const float n0 = t0 - t1;
             const float t2 = (-b) * e, t3 = a * f;
             const float n1 = t2 + t3;
-            y0[p] = n0 / det;
-            y1[p] = n1 / det;
+            st_out(y0 + p, n0 / det, true);
+            st_out(y1 + p, n1 / det, true);
         } else {
-            y0[p] = e;
-            y1[p] = f;
+            st_out(y0 + p, e, true);
+            st_out(y1 + p, f, true);
         }
     }
 }

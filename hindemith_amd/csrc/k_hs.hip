@@ -411,8 +411,8 @@ __global__ __launch_bounds__(256) void hs_copy2_kernel(const float *__restrict__
                                                        float *__restrict__ wv, int64_t n) {
     const int64_t stride = (int64_t)gridDim.x * 256;
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
-        wu[i] = u[i];
-        wv[i] = v[i];
+        st_out(wu + i, u[i], true);
+        st_out(wv + i, v[i], true);
     }
 }
 

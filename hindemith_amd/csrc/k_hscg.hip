@@ -52,8 +52,8 @@ __global__ __launch_bounds__(256) void hs_precond_kernel(
             y0[p] = n0 / det;
             y1[p] = n1 / det;
         } else {
-            y0[p] = e;
-            y1[p] = f;
+            st_out(y0 + p, e, true);
+            st_out(y1 + p, f, true);
         }
     }
 }
@@ -132,8 +132,8 @@ __global__ __launch_bounds__(NT) void hs_powers_kernel(PowArgs a) {
             const float u = a.u0[g], v = a.v0[g];
             win[0][c] = make_float2(u, v);
             if (a.copy_r && gx >= x0 && gx < x1 && gy >= y0 && gy < y1) {
-                a.V0[g] = u;
-                a.V1[g] = v;
+                st_out(a.V0 + g, u, rows_whole_lines(a.W));
+                st_out(a.V1 + g, v, rows_whole_lines(a.W));
             }
         }
     }

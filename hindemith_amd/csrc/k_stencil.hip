@@ -89,7 +89,7 @@ __global__ __launch_bounds__(256) void stencil2d_cols_kernel(const float *__rest
     }
 #pragma unroll
     for (int v = 0; v < ST_V; ++v)
-        if (r0 + v < H) st_out(y + (int64_t)(r0 + v) * W + c, acc[v]);
+        if (r0 + v < H) st_out(y + (int64_t)(r0 + v) * W + c, acc[v], rows_whole_lines(W));
 }
 
 // Wide stencils (radius > 4): taps straight from global memory (L1/L2 serve
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(256) void dia2d_kernel(const float *__restrict__ A,
                 }
             }
         }
-        st_out(y + p, acc);
+        st_out(y + p, acc, rows_whole_lines(W));
     }
 }
 
@@ -209,13 +209,13 @@ __global__ __launch_bounds__(256) void hs_setup_kernel(const float *__restrict__
         const int64_t p = (int64_t)r * W + c;
         float a1 = tap_apply(I1, H, W, r, c, dx);
         float a2 = tap_apply(I2, H, W, r, c, dx);
-        Ix[p] = a1 + a2;
+        st_out(Ix + p, a1 + a2, rows_whole_lines(W));
         float b1 = tap_apply(I1, H, W, r, c, dy);
         float b2 = tap_apply(I2, H, W, r, c, dy);
-        Iy[p] = b1 + b2;
+        st_out(Iy + p, b1 + b2, rows_whole_lines(W));
         float t2 = tap_apply(I2, H, W, r, c, dt);
         float t1 = tap_apply(I1, H, W, r, c, dt);
-        It[p] = t2 - t1;
+        st_out(It + p, t2 - t1, rows_whole_lines(W));
     }
 }
 
@@ -266,12 +266,12 @@ __global__ __launch_bounds__(256) void hs_setup4_kernel(const float *__restrict_
         const float b2 = tap_apply4(I2, H, W, r, c, dy);
         const float t2 = tap_apply4(I2, H, W, r, c, dt);
         const float t1 = tap_apply4(I1, H, W, r, c, dt);
-        Ix[p] = a1 + a2;
-        Iy[p] = b1 + b2;
-        It[p] = t2 - t1;
+        st_out(Ix + p, a1 + a2, rows_whole_lines(W));
+        st_out(Iy + p, b1 + b2, rows_whole_lines(W));
+        st_out(It + p, t2 - t1, rows_whole_lines(W));
         if (COPY) {
-            wu[p] = u[p];
-            wv[p] = v[p];
+            st_out(wu + p, u[p], rows_whole_lines(W));
+            st_out(wv + p, v[p], rows_whole_lines(W));
         }
     }
 }
@@ -335,12 +335,12 @@ __global__ __launch_bounds__(256) void hs_setup_cube_kernel(
         const float f2 = cube_apply(dy, b0[q], b1[q], b0[q + 1], b1[q + 1]);
         const float g2 = cube_apply(dt, b0[q], b1[q], b0[q + 1], b1[q + 1]);
         const float g1 = cube_apply(dt, a0[q], a1[q], a0[q + 1], a1[q + 1]);
-        Ix[p] = e1 + e2;
-        Iy[p] = f1 + f2;
-        It[p] = g2 - g1;
+        st_out(Ix + p, e1 + e2, rows_whole_lines(W));
+        st_out(Iy + p, f1 + f2, rows_whole_lines(W));
+        st_out(It + p, g2 - g1, rows_whole_lines(W));
         if (COPY) {
-            wu[p] = u[p];
-            wv[p] = v[p];
+            st_out(wu + p, u[p], rows_whole_lines(W));
+            st_out(wv + p, v[p], rows_whole_lines(W));
         }
     }
 }

@@ -57,7 +57,7 @@ __global__ __launch_bounds__(256) void warp_img2d_kernel(const float *__restrict
     if (c >= W) return;
     for (int r = blockIdx.y; r < H; r += gridDim.y) {
         const int64_t p = (int64_t)r * W + c;
-        dst[p] = warp_pixel(src, offx[p], offy[p], r, c, H, W);
+        st_out(dst + p, warp_pixel(src, offx[p], offy[p], r, c, H, W), rows_whole_lines(W));
     }
 }
 

@@ -220,12 +220,20 @@ std::string generate(const ProgSpec &sp, std::string &err) {
           << ";\n";
         if (!emit_row(o, sp, nreg, i, err)) return "";
     }
+    // outputs are written through the L2 (system-scope stores) when the rows
+    // fill whole 128-byte lines, as the library's own kernels store theirs
+    if (!store_slot.empty()) o << "            const bool wt = (a.W & 31) == 0;\n";
     for (int i = 0; i < R; ++i) {
         if (store_slot.empty()) break;
         o << "            if (ybase + " << i << " < a.H) {\n";
-        for (size_t k = 0; k < store_slot.size(); ++k)
-            o << "                *(float *)((char *)(a.out[" << store_slot[k] << "] + (long long)y"
-              << i << " * a.W) + cb) = st" << k << "_" << i << ";\n";
+        for (size_t k = 0; k < store_slot.size(); ++k) {
+            const std::string ptr = "(float *)((char *)(a.out[" + std::to_string(store_slot[k]) +
+                                    "] + (long long)y" + std::to_string(i) + " * a.W) + cb)";
+            const std::string val = "st" + std::to_string(k) + "_" + std::to_string(i);
+            o << "                if (wt) __hip_atomic_store(" << ptr << ", " << val
+              << ", __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);\n"
+              << "                else *" << ptr << " = " << val << ";\n";
+        }
         o << "            }\n";
     }
     o << "        }\n    }\n";

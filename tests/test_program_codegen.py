@@ -89,7 +89,7 @@ def test_rows_per_thread():
     axpy.compile(a=np.float32(2.0), x=x, y=x.copy())
     src = axpy.program_source(compile=True)
     assert "const int y3 = min(ybase + 3, a.H - 1);" in src and "ybase = blockIdx.y * 4" in src
-    assert src.index("st0_3 = ") < src.index("*(float *)((char *)(a.out[0] + (long long)y0")
+    assert src.index("st0_3 = ") < src.index("__hip_atomic_store((float *)((char *)(a.out[0] + (long long)y0")
     z = np.zeros((24, 40), np.float32)
     hs_setup_and_residual.compile(I1=z, I2=z.copy(), Dx=Stencil(*TAPS["Dx"]),
                                   Dy=Stencil(*TAPS["Dy"]), Dt=Stencil(*TAPS["Dt"]),

@@ -31,6 +31,7 @@
 #include "hs_tile.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <utility>
 
@@ -108,31 +109,62 @@ struct AdjTbArgs {
 
 constexpr int ADJ_TW = 64, ADJ_TH = 32, ADJ_NT = 1024, ADJ_MMAX = 8;
 
+// 1/d rounded to nearest: the hardware reciprocal refined by one Newton step
+// (hs_recip, the IEEE value wherever it accepts d), the IEEE division for the
+// rest (a branch few lanes take)
+__device__ __forceinline__ float adj_recip(float d) {
+    float r = hs_recip(d);
+    if (r != r) {
+        float dd = d;
+        asm volatile("" : "+v"(dd));
+        r = 1.0f / dd;
+    }
+    return r;
+}
+
 // Compiled for 8 waves per SIMD: two 1024-thread work-groups per CU.
 // Thread layout: slots 0..PT-1 of a thread are tile pixels in tile order
 // (i = tid + j*NT), so the per-pixel accumulators and the step's (t, ubar,
 // vbar) exist only there and load coalesced; the remaining PH slots cover the
-// halo ring of the region.  The gradient g of every slot stays in registers
-// (only its own thread reads or writes it); only (g_ubar, g_vbar), which
-// pass B gathers from neighbours, lives in LDS, double-buffered, so pass B of
-// step s and pass A of step s+1 (same region) run back to back with one
-// barrier per step.
+// ring around the tile of the largest region (the tile grown by MMAX), laid
+// out statically so that a slot's cell follows from its index by constant
+// divisions.  The gradient g of every slot stays in registers (only its own
+// thread reads or writes it); only (g_ubar, g_vbar), which pass B gathers
+// from neighbours, lives in LDS, double-buffered, so pass B of step s and
+// pass A of step s+1 (same region) run back to back with one barrier per
+// step.  The step loop is unrolled over the two buffers, so each buffer's
+// offset is a constant.
 // NTAP > 0: the averaging stencil has exactly NTAP taps (straight-line
 // gather, its reads issued in groups of four so the latency of a group is
-// paid once); 0: any count up to HS_MAX_TAPS.
-template <int NT, int TW, int TH, int MMAX, int NTAP>
+// paid once); 0: any count up to HS_MAX_TAPS.  RING: the 8 taps sit at the
+// canonical offsets HS_RING_OX/OY in that order (weights at run time), so
+// every tap of a gather is a constant LDS offset from the cell's own address:
+// no address arithmetic per tap.  Same arithmetic in the same order either
+// way.
+template <int NT, int TW, int TH, int MMAX, int NTAP, bool RING>
 __global__ __launch_bounds__(NT, 8) void hs_adj_tb_kernel(AdjTbArgs a, Taps taps) {
+    static_assert(!RING || NTAP == 8, "the ring build has the 8 canonical taps");
     // LDS window: the tile grown by MMAX, plus one column and one row, so the
     // ghost gathers of pass B (up to two pixels past the image) stay inside
     // it on every side
     constexpr int PITCH = TW + 2 * MMAX + 1, RHM = TH + 2 * MMAX + 1, NPX = PITCH * RHM;
+    constexpr int BW = TW + 2 * MMAX, BH = TH + 2 * MMAX;  // the largest region
     constexpr int TP = TW * TH;
     static_assert(TP % NT == 0, "tile pixels must split evenly over the threads");
+    static_assert((TW & (TW - 1)) == 0, "tile width must be a power of two");
     constexpr int PT = TP / NT;
-    constexpr int PH = ((TW + 2 * MMAX) * (TH + 2 * MMAX) - TP + NT - 1) / NT;
+    constexpr int NTOP = MMAX * BW, NSIDE = TH * 2 * MMAX;
+    constexpr int PH = (BW * BH - TP + NT - 1) / NT;
     constexpr int P = PT + PH;
-    __shared__ float2 GB[2][NPX];  // (g_ubar, g_vbar) on the current region
+    // Window cell c of buffer b is GB[LEAD + b*NPX + c]: the LEAD cells in
+    // front keep the lowest tap of any gathered cell at a non-negative
+    // offset from the cell's own address
+    constexpr int LEAD = PITCH + 1;
+    static_assert(NPX * 8 < (1 << 15), "cell byte offsets must fit 15 bits");
+    static_assert(MMAX <= 15, "distances must fit 4 bits");
+    __shared__ float2 GB[LEAD + 2 * NPX];
 
+    const int tid = threadIdx.x;
     const int b = blockIdx.y;
     const int64_t off = (int64_t)b * a.bstride;
     const float alpha2 = a.alpha2_b[b];
@@ -141,135 +173,140 @@ __global__ __launch_bounds__(NT, 8) void hs_adj_tb_kernel(AdjTbArgs a, Taps taps
     const int tyi = t / a.tiles_x, txi = t - tyi * a.tiles_x;
     const int x0 = txi * TW, y0 = tyi * TH;
     const int x1 = min(x0 + TW, W), y1 = min(y0 + TH, H);
-    const int gx0 = max(x0 - m, 0), gy0 = max(y0 - m, 0);
-    const int gx1 = min(x1 + m, W), gy1 = min(y1 + m, H);
-    const int RW = gx1 - gx0;
     const bool clipped = (x0 - m < 0) || (y0 - m < 0) || (x1 + m > W) || (y1 + m > H);
     const int tw = x1 - x0, th = y1 - y0;
-    const int tx = x0 - gx0, ty = y0 - gy0;  // tile origin in region coordinates
-    // The LDS window's corner is pixel (x0 - MMAX, y0 - MMAX) for every tile;
-    // (sx, sy) takes region coordinates to window coordinates.
+    // the window's corner is pixel (x0 - MMAX, y0 - MMAX) for every tile
     const int wx0 = x0 - MMAX, wy0 = y0 - MMAX;
-    const int sx = gx0 - wx0, sy = gy0 - wy0;
-    // image borders the pixel at window cell (rx, ry) lies on: bit 0 left,
-    // 1 right, 2 top, 3 bottom
-    auto border = [&](int rx, int ry) {
-        const int gx = wx0 + rx, gy = wy0 + ry;
+    // pixels are addressed from the tile's corner pixel by byte offsets that
+    // fit 32 bits (H, W < 32768): unsigned for the tile's own pixels (the
+    // load's uniform base plus one offset register), signed for the ring
+    const int64_t corner = off + (int64_t)y0 * W + x0;
+    const bool wt = rows_whole_lines(W);
+    // tile slot j's pixel is NT / TW rows below slot 0's: one offset register
+    // (slot 0's) serves every slot, from a per-slot uniform base
+    auto at_c = [&](const float *base, int j = 0) {
+        return reinterpret_cast<const char *>(base + corner + (int64_t)j * (NT / TW) * W);
+    };
+    auto ld = [](const char *p, unsigned o) { return *reinterpret_cast<const float *>(p + o); };
+
+    // Every slot's cell: pk = D << 27 | E << 15 | A, where A is 8 x the window
+    // cell index (the cell's byte offset in a buffer), E the image borders the
+    // pixel lies on (bit 0 left, 1 right, 2 top, 3 bottom) and D its distance
+    // outside the tile (-1: no pixel).  The region of reach e (the tile grown
+    // by e, clipped to the image) holds the slot exactly when D <= e, i.e.
+    // when (unsigned)pk < (e + 1) << 27: one compare.
+    auto border = [&](int gx, int gy) {
         return (gx == 0) | (gx == W - 1) << 1 | (gy == 0) << 2 | (gy == H - 1) << 3;
     };
-
-    // Every slot's window coordinates, its distance D outside the tile and
-    // its border flags E, packed D << 27 | ry << 16 | E << 12 | rx (-1: no
-    // pixel).  The region of reach e (the tile grown by e, clipped to the
-    // image) holds the slot exactly when D <= e, i.e. when (unsigned)pk <
-    // (e + 1) << 27: one compare.
     int pk[P];
-#pragma unroll
-    for (int j = 0; j < PT; ++j) {
-        const int i = threadIdx.x + j * NT;
-        const int ly = i / TW, lx = i - ly * TW;
-        const int rx = tx + lx + sx, ry = ty + ly + sy;
-        pk[j] = (lx < tw && ly < th) ? (ry << 16) | (border(rx, ry) << 12) | rx : -1;  // D = 0
-    }
-    {
-        const int lw = tx, rw = gx1 - x1, sw = lw + rw;
-        const int n_top = ty * RW, n_bot = (gy1 - y1) * RW, n_side = th * sw;
-#pragma unroll
-        for (int j = 0; j < PH; ++j) {
-            const int h = threadIdx.x + j * NT;
-            int rx = -1, ry = 0;
-            if (h < n_top) {
-                ry = h / RW;
-                rx = h - ry * RW;
-            } else if (h < n_top + n_bot) {
-                const int q = h - n_top;
-                ry = ty + th + q / RW;
-                rx = q % RW;
-            } else if (h < n_top + n_bot + n_side) {
-                const int q = h - n_top - n_bot;
-                const int r = q / sw, c = q - r * sw;
-                ry = ty + r;
-                rx = c < lw ? c : tx + tw + (c - lw);
-            }
-            if (rx < 0) {
-                pk[PT + j] = -1;
-            } else {
-                const int gx = gx0 + rx, gy = gy0 + ry;
-                const int dx = max(max(x0 - gx, gx - x1 + 1), 0);
-                const int dy = max(max(y0 - gy, gy - y1 + 1), 0);
-                const int wx = rx + sx, wy = ry + sy;
-                pk[PT + j] = (max(dx, dy) << 27) | (wy << 16) | (border(wx, wy) << 12) | wx;
-            }
-        }
-    }
-    auto in_reach = [](int pkj, int e) { return (unsigned)pkj < ((unsigned)e + 1u) << 27; };
-    const HsTapTable<PITCH> tp(taps, NTAP);
-
-    const float *Ix = a.Ix + off, *Iy = a.Iy + off;
+    unsigned ot = (unsigned)(tid / TW) * (unsigned)W * 4u + (unsigned)(tid % TW) * 4u;
     float cx[P], cy[P], cd[P];
     float2 g[P];
     float ax[PT], ay[PT], at[PT], aa[PT], ct[PT];
 #pragma unroll
-    for (int j = 0; j < P; ++j) {
-        cx[j] = cy[j] = 0.0f;
-        cd[j] = 1.0f;
-        g[j] = make_float2(0.0f, 0.0f);
-        if (j < PT) ax[j] = ay[j] = at[j] = aa[j] = ct[j] = 0.0f;
-        if (pk[j] < 0) continue;
-        const int rx = pk[j] & 0xfff, ry = (pk[j] >> 16) & 0x7ff;
-        const int64_t q = (int64_t)(wy0 + ry) * W + wx0 + rx;
-        const float ix = Ix[q], iy = Iy[q];
-        cx[j] = ix;
-        cy[j] = iy;
-        cd[j] = 1.0f / ((alpha2 + ix * ix) + iy * iy);  // reciprocal of den
-        g[j] = make_float2(a.gu_in[off + q], a.gv_in[off + q]);
-        // the accumulators continue from their stored value and take one
-        // step's term at a time, so the sums do not depend on how the
-        // reverse sweep is cut into launches
-        if (j < PT) {
-            ax[j] = a.gIx[off + q];
-            ay[j] = a.gIy[off + q];
-            at[j] = a.gIt[off + q];
-            aa[j] = a.ga2[off + q];
-            ct[j] = a.It[off + q];
+    for (int j = 0; j < PT; ++j) {
+        const int i = tid + j * NT;
+        const int ly = i / TW, lx = i % TW;
+        const char *const pIx = at_c(a.Ix, j), *const pIy = at_c(a.Iy, j);
+        pk[j] = -1;
+        if (lx < tw && ly < th) {
+            pk[j] = (border(x0 + lx, y0 + ly) << 15) |
+                    (((ly + MMAX) * PITCH + lx + MMAX) * 8);  // D = 0
+            const float ix = ld(pIx, ot), iy = ld(pIy, ot);
+            cx[j] = ix;
+            cy[j] = iy;
+            cd[j] = adj_recip((alpha2 + ix * ix) + iy * iy);
+            g[j] = make_float2(ld(at_c(a.gu_in, j), ot), ld(at_c(a.gv_in, j), ot));
+            // the accumulators continue from their stored value and take one
+            // step's term at a time, so the sums do not depend on how the
+            // reverse sweep is cut into launches
+            ax[j] = ld(at_c(a.gIx, j), ot);
+            ay[j] = ld(at_c(a.gIy, j), ot);
+            at[j] = ld(at_c(a.gIt, j), ot);
+            aa[j] = ld(at_c(a.ga2, j), ot);
+            ct[j] = ld(at_c(a.It, j), ot);
         }
     }
+#pragma unroll
+    for (int j = 0; j < PH; ++j) {
+        // the ring: MMAX rows above the tile, MMAX below (BW wide each), then
+        // MMAX columns on each side of the tile's TH rows
+        const int h = tid + j * NT;
+        int wx = -1, wy = 0;
+        if (h < NTOP) {
+            wy = h / BW;
+            wx = h - wy * BW;
+        } else if (h < 2 * NTOP) {
+            const int q = h - NTOP, r = q / BW;
+            wy = MMAX + TH + r;
+            wx = q - r * BW;
+        } else if (h < 2 * NTOP + NSIDE) {
+            const int q = h - 2 * NTOP, r = q / (2 * MMAX), c = q - r * (2 * MMAX);
+            wy = MMAX + r;
+            wx = c < MMAX ? c : TW + c;
+        }
+        const int gx = wx0 + wx, gy = wy0 + wy;
+        pk[PT + j] = -1;
+        if (wx >= 0 && gx >= 0 && gx < W && gy >= 0 && gy < H) {
+            const int dx = max(max(x0 - gx, gx - x1 + 1), 0);
+            const int dy = max(max(y0 - gy, gy - y1 + 1), 0);
+            const int d = max(dx, dy);
+            // only cells within m of the tile are ever in reach
+            if (d <= m) {
+                pk[PT + j] = (d << 27) | (border(gx, gy) << 15) | ((wy * PITCH + wx) * 8);
+                const int o = ((gy - y0) * W + (gx - x0)) * 4;
+                const float ix = *reinterpret_cast<const float *>(at_c(a.Ix) + o);
+                const float iy = *reinterpret_cast<const float *>(at_c(a.Iy) + o);
+                cx[PT + j] = ix;
+                cy[PT + j] = iy;
+                cd[PT + j] = adj_recip((alpha2 + ix * ix) + iy * iy);
+                g[PT + j] = make_float2(*reinterpret_cast<const float *>(at_c(a.gu_in) + o),
+                                        *reinterpret_cast<const float *>(at_c(a.gv_in) + o));
+            }
+        }
+    }
+    const HsTapTable<PITCH> tp(taps, NTAP);
+
+    // the cell at byte offset A (from buffer 0's first cell) displaced by c
+    // cells
+    auto cell = [&](int A, int c) -> float2 & {
+        return *reinterpret_cast<float2 *>(reinterpret_cast<char *>(GB) + A + c * 8);
+    };
 
     // (ubar, vbar) of forward step hi - step at the tile slots; its t is
     // recomputed as ((Ix ubar + Iy vbar) + It) / den (with the reciprocal:
     // within two ulps of the forward's quotient)
     float lu[PT], lv[PT];
     auto load_lin = [&](int step) {
-        const float *U0 = a.lin_hi - (int64_t)(step - 1) * a.lin_step + off;
-        const float *V0 = U0 + a.plane;
+        const float *U0 = a.lin_hi - (int64_t)(step - 1) * a.lin_step;
+        // (opaque per step: otherwise each address becomes a 64-bit induction
+        // variable held across the loop)
+        asm volatile("" : "+v"(ot));
 #pragma unroll
         for (int j = 0; j < PT; ++j) {
-            int pkj = pk[j];
-            asm volatile("" : "+v"(pkj));  // recompute the offsets per step
-            lu[j] = lv[j] = 0.0f;
-            if (pkj >= 0) {
-                const int64_t q =
-                    (int64_t)(wy0 + ((pkj >> 16) & 0x7ff)) * W + wx0 + (pkj & 0xfff);
-                lu[j] = U0[q];
-                lv[j] = V0[q];
+            if (pk[j] >= 0) {
+                lu[j] = ld(at_c(U0, j), ot);
+                lv[j] = ld(at_c(U0 + a.plane, j), ot);
             }
         }
     };
 
-    // pass A of step `step` on R_{m-step+1}: s, (g_ubar, g_vbar) into GB[buf]
-    // and the tile's accumulator terms (the alpha term -s t included)
-    auto pass_a = [&](int step, int buf) {
-        const int e = m - step + 1;
+    // pass A of step `step` on R_{m-step+1}: s, (g_ubar, g_vbar) into buffer
+    // BUF and the tile's accumulator terms (the alpha term -s t included)
+    auto pass_a = [&](int step, auto buf_c) {
+        constexpr int BUF = decltype(buf_c)::value;
+        const unsigned lim = (unsigned)(m - step + 2) << 27;
 #pragma unroll
         for (int j = 0; j < P; ++j) {
-            int pkj = pk[j];
-            asm volatile("" : "+v"(pkj));
-            if (!in_reach(pkj, e)) continue;
-            const int rx = pkj & 0xfff, ry = (pkj >> 16) & 0x7ff;
+            asm volatile("" : "+v"(pk[j]));  // the address is recomputed per step
+            const int pkj = pk[j];
+            if ((unsigned)pkj >= lim) continue;
+            const int A = pkj & 0x7fff;
             // (fused multiply-adds: the adjoint is checked against fp64 and
             // finite differences, not bit for bit)
             const float s = -fmaf(cx[j], g[j].x, cy[j] * g[j].y) * cd[j];
-            GB[buf][ry * PITCH + rx] = make_float2(fmaf(cx[j], s, g[j].x), fmaf(cy[j], s, g[j].y));
+            cell(A, LEAD + BUF * NPX) =
+                make_float2(fmaf(cx[j], s, g[j].x), fmaf(cy[j], s, g[j].y));
             if (j < PT) {  // a tile pixel (always inside the region)
                 const float tt = (fmaf(cx[j], lu[j], cy[j] * lv[j]) + ct[j]) * cd[j];
                 const float st2 = 2.0f * s * tt;
@@ -281,109 +318,140 @@ __global__ __launch_bounds__(NT, 8) void hs_adj_tb_kernel(AdjTbArgs a, Taps taps
         }
     };
 
-    // Window cells outside the image read as zero in both buffers: pass B's
-    // unclamped gathers then take no term from a source that does not exist.
-    // (Pass A writes only image pixels, so the zeros stay.)
-    if (clipped) {
-        for (int c = threadIdx.x; c < NPX; c += NT) {
-            const int ry = c / PITCH, rx = c - ry * PITCH;
-            const int gx = wx0 + rx, gy = wy0 + ry;
-            if (gx < 0 || gx >= W || gy < 0 || gy >= H)
-                GB[0][c] = GB[1][c] = make_float2(0.0f, 0.0f);
-        }
-    }
-
-    load_lin(1);
-    pass_a(1, 0);
-    __syncthreads();
-
-    for (int step = 1; step <= m; ++step) {
-        const int buf = (step - 1) & 1;
-        // pass B on R_{m-step}: g <- A^T g_b.  A source q whose tap lands
-        // outside the image clamps onto the border pixel p: in the forward
-        // map that is a read of a ghost of p (the cell across the border,
-        // which holds p's value), so p also collects the unclamped gather at
-        // each of its ghosts (one per border it lies on, three at a corner).
-        // The zeros outside the image drop the sources that do not exist.
-        const int e = m - step;
-        const float2 *GBb = GB[buf];
-        auto gather = [&](const float2 *gb, float &au, float &av) {
-            if (NTAP > 0) {
-                constexpr int G = 4;
+    // A^T applied at the cell displaced by c0 - LEAD cells from A: the sum
+    // over taps d of w_d times the cell at -(oy_d, ox_d) from it
+    auto gather = [&](int A, int c0, float &au, float &av) {
+        if (NTAP > 0) {
+            constexpr int G = 4;
 #pragma unroll
-                for (int d0 = 0; d0 < NTAP; d0 += G) {
-                    float2 q[G];
+            for (int d0 = 0; d0 < NTAP; d0 += G) {
+                float2 q[G];
 #pragma unroll
-                    for (int i = 0; i < G; ++i)
-                        if (d0 + i < NTAP) q[i] = gb[-tp.dl[d0 + i]];
-#pragma unroll
-                    for (int i = 0; i < G; ++i)
-                        if (d0 + i < NTAP) {
-                            au = fmaf(tp.w[d0 + i], q[i].x, au);
-                            av = fmaf(tp.w[d0 + i], q[i].y, av);
-                        }
-                    // the next group's reads stay behind this group's use
-                    asm volatile("" ::: "memory");
-                }
-            } else {
-#pragma unroll
-                for (int d = 0; d < HS_MAX_TAPS; ++d) {
-                    if (d < tp.nd) {
-                        const float2 q = gb[-tp.dl[d]];
-                        au = fmaf(tp.w[d], q.x, au);
-                        av = fmaf(tp.w[d], q.y, av);
+                for (int i = 0; i < G; ++i)
+                    if (d0 + i < NTAP) {
+                        const int dl = RING ? HS_RING_OY[(d0 + i) & 7] * PITCH +
+                                                  HS_RING_OX[(d0 + i) & 7]
+                                            : tp.dl[d0 + i];
+                        q[i] = cell(A, c0 - dl);
                     }
+#pragma unroll
+                for (int i = 0; i < G; ++i)
+                    if (d0 + i < NTAP) {
+                        au = fmaf(tp.w[d0 + i], q[i].x, au);
+                        av = fmaf(tp.w[d0 + i], q[i].y, av);
+                    }
+                // the next group's reads stay behind this group's use
+                asm volatile("" ::: "memory");
+            }
+        } else {
+#pragma unroll
+            for (int d = 0; d < HS_MAX_TAPS; ++d) {
+                if (d < tp.nd) {
+                    const float2 q = cell(A, c0 - tp.dl[d]);
+                    au = fmaf(tp.w[d], q.x, au);
+                    av = fmaf(tp.w[d], q.y, av);
                 }
             }
-        };
+        }
+    };
+
+    // pass B of step `step` on R_{m-step}: g <- A^T g_b, from buffer BUF.  A
+    // source q whose tap lands outside the image clamps onto the border pixel
+    // p: in the forward map that is a read of a ghost of p (the cell across
+    // the border, which holds p's value), so p also collects the unclamped
+    // gather at each of its ghosts (one per border it lies on, three at a
+    // corner).  The zeros outside the image drop the sources that do not
+    // exist.  CLIP (compile time): the tile's regions reach an image border.
+    auto pass_b = [&](int step, auto buf_c, auto clip_c) {
+        constexpr int BUF = decltype(buf_c)::value;
+        constexpr bool CLIP = decltype(clip_c)::value;
+        constexpr int C0 = LEAD + BUF * NPX;
+        const unsigned lim = (unsigned)(m - step + 1) << 27;
 #pragma unroll
         for (int j = 0; j < P; ++j) {
-            int pkj = pk[j];
-            asm volatile("" : "+v"(pkj));
-            if (!in_reach(pkj, e)) continue;
-            const int rx = pkj & 0xfff, ry = (pkj >> 16) & 0x7ff;
-            const float2 *gb = GBb + ry * PITCH + rx;
+            asm volatile("" : "+v"(pk[j]));
+            const int pkj = pk[j];
+            if ((unsigned)pkj >= lim) continue;
+            const int A = pkj & 0x7fff;
             float au = 0.0f, av = 0.0f;
-            gather(gb, au, av);
-            const int eb = (pkj >> 12) & 15;
-            if (clipped && eb) {
-                const bool l = eb & 1, r = eb & 2;
-                if (l) gather(gb - 1, au, av);
-                if (r) gather(gb + 1, au, av);
-                if (eb & 4) {
-                    gather(gb - PITCH, au, av);
-                    if (l) gather(gb - PITCH - 1, au, av);
-                    if (r) gather(gb - PITCH + 1, au, av);
-                }
-                if (eb & 8) {
-                    gather(gb + PITCH, au, av);
-                    if (l) gather(gb + PITCH - 1, au, av);
-                    if (r) gather(gb + PITCH + 1, au, av);
+            gather(A, C0, au, av);
+            if (CLIP) {
+                const int eb = (pkj >> 15) & 15;
+                if (eb) {
+                    const bool l = eb & 1, r = eb & 2;
+                    if (l) gather(A, C0 - 1, au, av);
+                    if (r) gather(A, C0 + 1, au, av);
+                    if (eb & 4) {
+                        gather(A, C0 - PITCH, au, av);
+                        if (l) gather(A, C0 - PITCH - 1, au, av);
+                        if (r) gather(A, C0 - PITCH + 1, au, av);
+                    }
+                    if (eb & 8) {
+                        gather(A, C0 + PITCH, au, av);
+                        if (l) gather(A, C0 + PITCH - 1, au, av);
+                        if (r) gather(A, C0 + PITCH + 1, au, av);
+                    }
                 }
             }
             g[j] = make_float2(au, av);
         }
-        if (step < m) {
-            // (read after the gather: holding them across it would spill)
-            load_lin(step + 1);
-            // GB[buf ^ 1] was last read in pass B of step - 1, before the
-            // previous barrier
-            pass_a(step + 1, buf ^ 1);
-            __syncthreads();
+    };
+
+    // Window cells outside the image read as zero in both buffers: pass B's
+    // unclamped gathers then take no term from a source that does not exist.
+    // (Pass A writes only image pixels, so the zeros stay.)
+    if (clipped) {
+        for (int c = tid; c < NPX; c += NT) {
+            const int ry = c / PITCH, rx = c - ry * PITCH;
+            const int gx = wx0 + rx, gy = wy0 + ry;
+            if (gx < 0 || gx >= W || gy < 0 || gy >= H)
+                GB[LEAD + c] = GB[LEAD + NPX + c] = make_float2(0.0f, 0.0f);
         }
     }
 
+    using B0 = std::integral_constant<int, 0>;
+    using B1 = std::integral_constant<int, 1>;
+    load_lin(1);
+    pass_a(1, B0());
+    __syncthreads();
+    auto run = [&](auto clip_c) {
+        // step s reads the buffer pass A of step s wrote, (s - 1) & 1; the
+        // other one was last read in pass B of step s - 1, before the
+        // previous barrier
+        // The next step's (ubar, vbar) are read before the gather, whose
+        // work then covers their latency (read after it, every wave waited
+        // for them right after the gather: each step paid a full memory
+        // latency behind its barrier).
+        for (int step = 1; step <= m; step += 2) {
+            if (step < m) load_lin(step + 1);
+            pass_b(step, B0(), clip_c);
+            if (step == m) break;
+            pass_a(step + 1, B1());
+            __syncthreads();
+            if (step + 1 < m) load_lin(step + 2);
+            pass_b(step + 1, B1(), clip_c);
+            if (step + 1 == m) break;
+            pass_a(step + 2, B0());
+            __syncthreads();
+        }
+    };
+    if (clipped)
+        run(std::true_type());
+    else
+        run(std::false_type());
+
+    auto st = [&](float *base, int j, float x) {
+        st_out(reinterpret_cast<float *>(const_cast<char *>(at_c(base, j)) + ot), x, wt);
+    };
 #pragma unroll
     for (int j = 0; j < PT; ++j) {
         if (pk[j] < 0) continue;
-        const int rx = pk[j] & 0xfff, ry = (pk[j] >> 16) & 0x7ff;
-        const int64_t q = off + (int64_t)(wy0 + ry) * W + wx0 + rx;
-        st_out(a.gu_out + q, g[j].x, rows_whole_lines(a.W));
-        st_out(a.gv_out + q, g[j].y, rows_whole_lines(a.W));
-        st_out(a.gIx + q, ax[j], rows_whole_lines(a.W));
-        st_out(a.gIy + q, ay[j], rows_whole_lines(a.W));
-        st_out(a.gIt + q, at[j], rows_whole_lines(a.W));
-        st_out(a.ga2 + q, aa[j], rows_whole_lines(a.W));
+        st(a.gu_out, j, g[j].x);
+        st(a.gv_out, j, g[j].y);
+        st(a.gIx, j, ax[j]);
+        st(a.gIy, j, ay[j]);
+        st(a.gIt, j, at[j]);
+        st(a.ga2, j, aa[j]);
     }
 }
 
@@ -613,6 +681,10 @@ extern "C" int hmx_hs_adjoint(const float *I1, const float *I2, int B, int H, in
     a.ntiles = (int)ntiles;
     a.bstride = (int64_t)H * W;
 
+    // the reverse sweep's ring build for the canonical 8-neighbour stencil
+    // (HMX_ADJ_GENERIC=1 takes the generic build, same results bit for bit)
+    const char *gen = std::getenv("HMX_ADJ_GENERIC");
+    const bool ring = hs_ring_taps(T.av) && !(gen && gen[0] == '1');
     const int nck = hmx_hs_ckpt_count(iters, ckpt_every);
     for (int c = nck - 1; c >= 0; --c) {
         const int len = (iters - c * ckpt_every) < ckpt_every ? iters - c * ckpt_every : ckpt_every;
@@ -643,12 +715,16 @@ extern "C" int hmx_hs_adjoint(const float *I1, const float *I2, int B, int H, in
             a.gv_in = g[cur][1];
             a.gu_out = g[cur ^ 1][0];
             a.gv_out = g[cur ^ 1][1];
-            if (T.av.nd == 8)
-                hipLaunchKernelGGL((hs_adj_tb_kernel<ADJ_NT, ADJ_TW, ADJ_TH, ADJ_MMAX, 8>),
-                                   dim3((unsigned)ntiles, B), dim3(ADJ_NT), 0, s, a, T.av);
+            const dim3 grid((unsigned)ntiles, B);
+            if (ring)
+                hipLaunchKernelGGL((hs_adj_tb_kernel<ADJ_NT, ADJ_TW, ADJ_TH, ADJ_MMAX, 8, true>),
+                                   grid, dim3(ADJ_NT), 0, s, a, T.av);
+            else if (T.av.nd == 8)
+                hipLaunchKernelGGL((hs_adj_tb_kernel<ADJ_NT, ADJ_TW, ADJ_TH, ADJ_MMAX, 8, false>),
+                                   grid, dim3(ADJ_NT), 0, s, a, T.av);
             else
-                hipLaunchKernelGGL((hs_adj_tb_kernel<ADJ_NT, ADJ_TW, ADJ_TH, ADJ_MMAX, 0>),
-                                   dim3((unsigned)ntiles, B), dim3(ADJ_NT), 0, s, a, T.av);
+                hipLaunchKernelGGL((hs_adj_tb_kernel<ADJ_NT, ADJ_TW, ADJ_TH, ADJ_MMAX, 0, false>),
+                                   grid, dim3(ADJ_NT), 0, s, a, T.av);
             HMX_LAUNCH_CHECK("hs_adj_tb");
             cur ^= 1;
             hi -= mm;

@@ -103,6 +103,22 @@ def test_checkpoint_interval_invariance():
             assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("shape", [(100, 200), (37, 66)])
+def test_ring_build_matches_generic_build_bitwise(shape, monkeypatch):
+    """The reverse sweep's build for the canonical 8-neighbour stencil (every
+    tap a constant LDS offset) and its generic build (taps from the table)
+    perform the same arithmetic in the same order: identical results.  At
+    100 x 200 one tile runs the unclipped loop, the rest the clipped one; 21
+    steps in segments of 10 cover reverse launches of 8, 2 and 1 steps."""
+    H, W = shape
+    I1, I2, fx, fy, alpha = batch(2, H, W, 6)
+    ring = run_grad(I1, I2, fx, fy, alpha, 21, 10)
+    monkeypatch.setenv("HMX_ADJ_GENERIC", "1")
+    generic = run_grad(I1, I2, fx, fy, alpha, 21, 10)
+    for a, b in zip(ring, generic):
+        assert np.array_equal(a, b)
+
+
 @pytest.fixture(scope="module")
 def config5():
     """8 pairs at 512x512, 200 iterations, checkpoints every 20."""

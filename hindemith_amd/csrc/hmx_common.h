@@ -51,6 +51,29 @@ struct Taps {
 // Fill a Taps struct from host arrays; validates nd.
 int make_taps(Taps &t, int nd, const float *w, const int *ox, const int *oy);
 
+// A 4-tap stencil inside the derivative cube: every offset in {0, 1}^2 (the
+// canonical Dx, Dy, Dt), which reads only the 2 x 2 block at (c, r).
+__host__ __device__ inline bool taps_in_cube(const Taps &t) {
+    if (t.nd != 4) return false;
+    for (int d = 0; d < 4; ++d)
+        if (t.ox[d] < 0 || t.ox[d] > 1 || t.oy[d] < 0 || t.oy[d] > 1) return false;
+    return true;
+}
+
+// Such a stencil applied to the block x00 = X(c, r), x10 = X(c+1, r),
+// x01 = X(c, r+1), x11 = X(c+1, r+1) (clamped): each tap a select on its
+// uniform offset, accumulated with fma in tap order from 0
+__device__ __forceinline__ float cube_apply(const Taps &t, float x00, float x10, float x01,
+                                            float x11) {
+    float acc = 0.0f;
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+        const float lo = t.ox[d] ? x10 : x00, hi = t.ox[d] ? x11 : x01;
+        acc = fmaf(t.w[d], t.oy[d] ? hi : lo, acc);
+    }
+    return acc;
+}
+
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
 __host__ __device__ inline int clampi(int v, int lo, int hi) {

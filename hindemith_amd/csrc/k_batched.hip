@@ -126,6 +126,65 @@ __device__ __forceinline__ float setup_tap4(const float *__restrict__ x, int H, 
     return acc;
 }
 
+// Derivative setup of tile w of the list for stencils inside the cube
+// (taps_in_cube): thread tid takes column tid % TW of the tile and RPT
+// consecutive rows, and reads the (RPT + 1)-row, 2-column neighbourhood of
+// each frame once (row r+1 of one output row is row r of the next), 6 reads
+// per pixel instead of 24; each tap is then a register select, accumulated in
+// tap order from 0 exactly as setup_tile does, so the results are the same.
+template <int NT, int TW, int TH>
+__device__ __forceinline__ void setup_tile_cube(const BatchArgs &b, const SetupTaps &st, int w) {
+    static_assert(NT % TW == 0 && (TW * TH) % NT == 0, "whole columns of rows per thread");
+    constexpr int RPT = TW * TH / NT;
+    const PairDesc &P = b.tiles[w];
+    const int t = w - P.tile_base;
+    const int tyi = t / P.tiles_x, txi = t - tyi * P.tiles_x;
+    const int c = txi * TW + (int)(threadIdx.x % TW);
+    const int r0 = tyi * TH + (int)(threadIdx.x / TW) * RPT;
+    if (c >= P.W || r0 >= P.H) return;
+    const int c1 = min(c + 1, P.W - 1);
+    const bool wt = rows_whole_lines(P.W);
+    // every read is issued before the first store (the descriptor's pointers
+    // may alias as far as the compiler knows, so a read after a store would
+    // wait for it)
+    float a0[RPT + 1], a1[RPT + 1], b0[RPT + 1], b1[RPT + 1], uu[RPT], vv[RPT];
+#pragma unroll
+    for (int j = 0; j <= RPT; ++j) {
+        const int64_t row = (int64_t)min(r0 + j, P.H - 1) * P.W;
+        a0[j] = P.I1[row + c];
+        a1[j] = P.I1[row + c1];
+        b0[j] = P.I2[row + c];
+        b1[j] = P.I2[row + c1];
+    }
+    if (b.phase0) {  // the first phase reads the start state from (wu, wv)
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int64_t p = (int64_t)min(r0 + q, P.H - 1) * P.W + c;
+            uu[q] = P.u[p];
+            vv[q] = P.v[p];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const int r = r0 + q;
+        if (r >= P.H) break;
+        const int64_t p = (int64_t)r * P.W + c;
+        if (b.phase0) {
+            st_out(P.wu + p, uu[q], wt);
+            st_out(P.wv + p, vv[q], wt);
+        }
+        const float e1 = cube_apply(st.dx, a0[q], a1[q], a0[q + 1], a1[q + 1]);
+        const float e2 = cube_apply(st.dx, b0[q], b1[q], b0[q + 1], b1[q + 1]);
+        const float f1 = cube_apply(st.dy, a0[q], a1[q], a0[q + 1], a1[q + 1]);
+        const float f2 = cube_apply(st.dy, b0[q], b1[q], b0[q + 1], b1[q + 1]);
+        const float g2 = cube_apply(st.dt, b0[q], b1[q], b0[q + 1], b1[q + 1]);
+        const float g1 = cube_apply(st.dt, a0[q], a1[q], a0[q + 1], a1[q + 1]);
+        st_out(P.Ix + p, e1 + e2, wt);
+        st_out(P.Iy + p, f1 + f2, wt);
+        st_out(P.It + p, g2 - g1, wt);
+    }
+}
+
 // Derivative setup of tile w of the list (Ix = Dx*I1 + Dx*I2, ...), one
 // rounding per op.  TAP4: all three stencils have 4 taps (the canonical
 // cube), taken by the unrolled form.
@@ -246,10 +305,18 @@ __global__ __launch_bounds__(NT, WPE) void hs_batched_kernel(BatchArgs b, SetupT
 // addresses, read-ahead, reciprocal quotient).  One work-group per tile; the
 // grid is the whole list, dealt so each XCD gets a contiguous range.
 
+// The setup's work-group size, whatever the phase kernel's: a tile's setup is
+// a short streaming pass, and 1024-thread work-groups with two pixels per
+// thread spent more time being dispatched than moving data (64 uneven pairs:
+// 195 us; 256 threads with eight rows each read more per thread in flight).
+constexpr int kSetupNT = 256;
+
 template <int NT, int TW, int TH>
 __global__ __launch_bounds__(NT) void hs_multi_setup_kernel(BatchArgs b, SetupTaps st) {
     const int w = xcd_remap(blockIdx.x, b.total_tiles);
-    if (st.dx.nd == 4 && st.dy.nd == 4 && st.dt.nd == 4)
+    if (taps_in_cube(st.dx) && taps_in_cube(st.dy) && taps_in_cube(st.dt))
+        setup_tile_cube<NT, TW, TH>(b, st, w);
+    else if (st.dx.nd == 4 && st.dy.nd == 4 && st.dt.nd == 4)
         setup_tile<NT, TW, TH, true>(b, st, w);
     else
         setup_tile<NT, TW, TH>(b, st, w);
@@ -296,7 +363,7 @@ struct BVariant {
      nullptr, {nullptr, nullptr}}
 // stepped form (DB: double-buffered window, FDIV: reciprocal quotient)
 #define HB_S(NT, TW, TH, K, DB, FDIV)                                                     \
-    {NT, TW, TH, K, 0, {nullptr, nullptr}, hs_multi_setup_kernel<NT, TW, TH>,            \
+    {NT, TW, TH, K, 0, {nullptr, nullptr}, hs_multi_setup_kernel<kSetupNT, TW, TH>,      \
      {hs_multi_phase_kernel<NT, TW, TH, K, 8, DB, FDIV>,                                  \
       hs_multi_phase_kernel<NT, TW, TH, K, 0, DB, FDIV>}}
 static const BVariant kBVariants[] = {
@@ -375,7 +442,7 @@ static int launch_batch(const BatchPlan &P, const SetupTaps &st, const Taps &av,
         return HMX_OK;
     }
     const dim3 grid(P.total_tiles), block(V.nt);
-    hipLaunchKernelGGL(V.setup, grid, block, 0, s, a, st);
+    hipLaunchKernelGGL(V.setup, grid, dim3(kSetupNT), 0, s, a, st);
     HMX_LAUNCH_CHECK("hs_multi_setup");
     for (int ph = 0; ph < nph; ++ph) {
         const int kk = std::min(P.k, iters - ph * P.k);

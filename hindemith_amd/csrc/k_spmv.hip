@@ -68,10 +68,62 @@ static int launch_csr_L(int L, int nrows, const int *rows, const int *indptr,
     return HMX_OK;
 }
 
+// One row per lane, staged: a wavefront takes 64 consecutive rows, whose
+// entries are one contiguous span of indices / data.  The wave reads the span
+// coalesced (lane l takes entries l, l + 64, ...), gathers x for each entry,
+// and puts the (data, x) pairs in LDS; then each lane runs its own row's
+// chain acc = fma(data, x, acc) from LDS in entry order -- the reference's
+// per-row sequential chain (SpmvCSR.cpp:32-38), so the result is the one
+// lane's bit for bit.  The per-entry read of the direct form waits for its
+// index before its x gather, entry after entry; here a lane's gathers are
+// independent and in flight together.  A wave whose span exceeds its LDS
+// share (CSRW_CAP entries) runs the direct loop.  Waves synchronise only with
+// themselves (no work-group barrier), so a wave past the last row just exits.
+constexpr int CSRW_NT = 256, CSRW_CAP = 640;
+
+__global__ __launch_bounds__(CSRW_NT) void csr_wave_rows_kernel(int nrows,
+                                                                const int *__restrict__ indptr,
+                                                                const int *__restrict__ indices,
+                                                                const float *__restrict__ data,
+                                                                const float *__restrict__ x,
+                                                                float *__restrict__ y) {
+    __shared__ float2 stage[CSRW_NT / 64][CSRW_CAP];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t row0 = ((int64_t)blockIdx.x * (CSRW_NT / 64) + wave) * 64;
+    if (row0 >= nrows) return;
+    const int64_t row = row0 + lane;
+    const bool has = row < nrows;
+    const int64_t last = row0 + 64 < nrows ? row0 + 64 : nrows;
+    const int base = indptr[row0];
+    const int span = indptr[last] - base;  // the same value in every lane
+    const int beg = has ? indptr[row] : 0, end = has ? indptr[row + 1] : 0;
+    float acc = 0.0f;
+    if (span <= CSRW_CAP) {
+        float2 *S = stage[wave];
+#pragma unroll 5
+        for (int k = lane; k < span; k += 64) {
+            const int j = base + k;
+            S[k] = make_float2(data[j], x[indices[j]]);
+        }
+        // the wave's LDS writes before its reads (a wavefront's LDS accesses
+        // are performed in order; the fences keep the compiler's order)
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int j = beg; j < end; ++j) {
+            const float2 q = S[j - base];
+            acc = fmaf(q.x, q.y, acc);
+        }
+    } else {
+        for (int j = beg; j < end; ++j) acc = fmaf(data[j], x[indices[j]], acc);
+    }
+    if (has) y[row] = acc;
+}
+
 // Lanes per row from the mean row length: short rows (<= 8 nnz) go one row
-// per lane, the reference's own sequential chain, whose neighbouring rows
-// make each load instruction cover a contiguous span (2048^2 5-point
-// operator: 55.6 us, against 94 us at 8 lanes); longer rows get the largest
+// per lane, the reference's own sequential chain, staged through LDS by
+// csr_wave_rows_kernel (2048^2 5-point operator: 55.6 us for the direct
+// one-lane loop, against 94 us at 8 lanes); longer rows get the largest
 // power of two <= mean / 2, at most 64.
 int csr_auto_lanes(int M, int64_t nnz) {
     const double mean = M > 0 ? (double)nnz / M : 0.0;
@@ -84,6 +136,15 @@ int csr_auto_lanes(int M, int64_t nnz) {
 int launch_csr(int M, int64_t nnz, const int *indptr, const int *indices, const float *data,
                const float *x, float *y, int L, hipStream_t s) {
     if (L <= 0) L = csr_auto_lanes(M, nnz);
+    if (L == 1 && M > 0) {
+        const int64_t blocks = ((int64_t)M + CSRW_NT - 1) / CSRW_NT;
+        if (blocks <= 0x7fffffff) {
+            hipLaunchKernelGGL(csr_wave_rows_kernel, dim3((unsigned)blocks), dim3(CSRW_NT), 0, s,
+                               M, indptr, indices, data, x, y);
+            HMX_LAUNCH_CHECK("csr_wave_rows");
+            return HMX_OK;
+        }
+    }
     return launch_csr_L(L, M, nullptr, indptr, indices, data, x, y, s);
 }
 

@@ -286,24 +286,7 @@ __global__ __launch_bounds__(256) void hs_setup4_kernel(const float *__restrict_
 // 4 (CUBE_V + 1) / CUBE_V.
 constexpr int CUBE_V = 8;
 
-bool taps_in_cube(const Taps &t) {
-    if (t.nd != 4) return false;
-    for (int d = 0; d < 4; ++d)
-        if (t.ox[d] < 0 || t.ox[d] > 1 || t.oy[d] < 0 || t.oy[d] > 1) return false;
-    return true;
-}
-
-// x00 = X(c, r), x10 = X(c+1, r), x01 = X(c, r+1), x11 = X(c+1, r+1), clamped
-__device__ __forceinline__ float cube_apply(const Taps &t, float x00, float x10, float x01,
-                                            float x11) {
-    float acc = 0.0f;
-#pragma unroll
-    for (int d = 0; d < 4; ++d) {
-        const float lo = t.ox[d] ? x10 : x00, hi = t.ox[d] ? x11 : x01;
-        acc = fmaf(t.w[d], t.oy[d] ? hi : lo, acc);
-    }
-    return acc;
-}
+// (taps_in_cube and cube_apply: hmx_common.h)
 
 template <bool COPY>
 __global__ __launch_bounds__(256) void hs_setup_cube_kernel(

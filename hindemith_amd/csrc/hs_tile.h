@@ -12,6 +12,20 @@
 
 namespace hmx {
 
+// The canonical 8-tap averaging stencil's offsets, in its tap order (edges,
+// then corners).  Builds for it (ring-ordered tiles, RING tile bodies, the
+// adjoint's reverse sweep) take each tap at a compile-time LDS offset; the
+// weights stay run-time values.
+constexpr int HS_RING_OX[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
+constexpr int HS_RING_OY[8] = {0, 0, -1, 1, -1, -1, 1, 1};
+
+__host__ __device__ inline bool hs_ring_taps(const Taps &t) {
+    if (t.nd != 8) return false;
+    for (int d = 0; d < 8; ++d)
+        if (t.ox[d] != HS_RING_OX[d] || t.oy[d] != HS_RING_OY[d]) return false;
+    return true;
+}
+
 // One image of the sweep: read (u_in, v_in), write (u_out, v_out).
 struct HsImage {
     const float *Ix, *Iy, *It, *den;  // den null: recompute from alpha2
@@ -132,9 +146,11 @@ struct HsWindow {
 // NTAP > 0: the stencil has exactly NTAP taps (straight-line code, batched LDS
 // reads); NTAP == 0: any count up to HS_MAX_TAPS, tested per tap.
 // LOUT: also write each step's (t, ubar, vbar) of the output tile to a.lt,
-// a.lub, a.lvb, staged through lin (3 x TW x TH floats of LDS) and stored in
-// tile order between the step's two barriers (a compile-time switch: the
-// extra work and LDS stay out of the kernels that do not need it).
+// a.lub, a.lvb (a compile-time switch: the extra work stays out of the
+// kernels that do not need it).  Two-barrier builds stage them through lin
+// (3 x TW x TH floats of LDS) and store them in tile order between the step's
+// barriers; double-buffered (RING) builds store them from registers as each
+// tile cell is computed, at a per-cell offset found once.
 // DB: uv holds two windows; step j reads window (j-1)&1 and writes its result
 // straight into the other, so a step needs one barrier and no new values are
 // held in registers across it.
@@ -151,6 +167,9 @@ struct HsWindow {
 // COH: u, v are read with system-scope loads (sc0 sc1), which the hardware
 // keeps coherent across the XCDs' L2 caches; for a launch whose tiles read
 // other tiles' output of the same launch (hs_persist_kernel).
+// RING: the stencil is the canonical 8 taps (hs_ring_taps), so tap d's LDS
+// delta is a compile-time constant: a cell's reads are one address register
+// plus immediate offsets, and the shared form needs no register per tap.
 template <bool COH>
 __device__ __forceinline__ float hs_ld_uv(const char *p) {
     if (COH)
@@ -170,13 +189,24 @@ __device__ __forceinline__ void hs_st_uv(char *p, float x, bool wt) {
 template <int NT, int TW, int TH, int KMAX, int NTAP, bool LOUT = false, bool DB = false,
           bool FDIV = false,
           bool SHARED = !LOUT && ((TW + 2 * KMAX) * (TH + 2 * KMAX) + NT - 1) / NT <= 8,
-          bool PF = SHARED && DB, bool COH = false>
+          bool PF = SHARED && DB, bool COH = false, bool RING = false>
 __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
                                         const HsTapTable<TW + 2 * KMAX> &tp, float2 *uv,
                                         float *lin = nullptr, int tid_in = -1) {
-    static_assert(!(LOUT && DB), "the linearisation output needs the two-barrier step");
+    static_assert(!(LOUT && DB) || RING,
+                  "the linearisation output of a double-buffered step needs the ring taps");
+    static_assert(!RING || NTAP == 8, "the ring taps are 8");
     constexpr bool PREFETCH = PF && SHARED && NTAP > 0 && !LOUT;
+    // linearisation output stored from registers (no LDS staging)
+    constexpr bool LREG = LOUT && DB;
     constexpr int PITCH = TW + 2 * KMAX, RHM = TH + 2 * KMAX;
+    // tap d's LDS delta oy*PITCH + ox
+    auto dl = [&](int d) -> int {
+        if constexpr (RING)
+            return HS_RING_OY[d & 7] * PITCH + HS_RING_OX[d & 7];
+        else
+            return tp.dl[d];
+    };
     constexpr int NPX = PITCH * RHM;
     constexpr int P = (NPX + NT - 1) / NT;
     // tid_in: the thread index as a caller that loops over tiles passes it,
@@ -338,6 +368,20 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
             if (in[j]) uv[tid + j * NT] = make_float2(ui[j], vi[j]);
         }
     }
+    // LREG: each slot's byte offset from the output tile's corner when its
+    // cell is a pixel of the output tile (stored every step), else ~0u
+    unsigned lo[LREG ? P : 1];
+    const int64_t lcorner = (int64_t)y0 * a.W + x0;
+    const bool lwt = rows_whole_lines(a.W);
+    if (LREG) {
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            const int lx = (pk[j] & 0xfff) - k, ly = ((pk[j] >> 16) & 0x7ff) - k;
+            const bool tile = pk[j] >= 0 && (unsigned)lx < (unsigned)(x1 - x0) &&
+                              (unsigned)ly < (unsigned)(y1 - y0);
+            lo[j] = tile ? ((unsigned)ly * (unsigned)a.W + (unsigned)lx) * 4u : ~0u;
+        }
+    }
     __syncthreads();
 
     // One Jacobi step.  PAR (compile time, SHARED builds) is the window step
@@ -353,6 +397,12 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
         const int par = SHARED ? PAR : (it - 1) & 1;
         const float2 *uvr = DB ? uv + par * NPX : uv;
         float2 *uvw = DB ? uv + (par ^ 1) * NPX : uv;
+        if (LREG) {
+            // (opaque per step: otherwise each store address becomes a
+            // 64-bit induction variable held across the loop)
+#pragma unroll
+            for (int j = 0; j < P; ++j) asm volatile("" : "+v"(lo[j]));
+        }
         float nu[P], nv[P];
         bool act[P];
         if (PREFETCH) {
@@ -364,15 +414,15 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
             const float2 *b0 = uvr + tid;
             float2 qa[NPF];
 #pragma unroll
-            for (int d = 0; d < NPF; ++d) qa[d] = b0[tp.dl[d]];
+            for (int d = 0; d < NPF; ++d) qa[d] = b0[dl(d)];
 #pragma unroll
             for (int j = 0; j < P; ++j) {
                 float2 qr[PREFETCH ? NTAP - NPF + 1 : 1], qn[NPF];
 #pragma unroll
-                for (int d = NPF; d < NTAP; ++d) qr[d - NPF] = b0[j * NT + tp.dl[d]];
+                for (int d = NPF; d < NTAP; ++d) qr[d - NPF] = b0[j * NT + dl(d)];
                 if (j + 1 < P) {
 #pragma unroll
-                    for (int d = 0; d < NPF; ++d) qn[d] = b0[(j + 1) * NT + tp.dl[d]];
+                    for (int d = 0; d < NPF; ++d) qn[d] = b0[(j + 1) * NT + dl(d)];
                 }
                 act[j] = pk[j] >= act_min;
                 if (act[j]) {
@@ -409,7 +459,7 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
 #pragma unroll
                 for (int d = 0; d < (NTAP ? NTAP : HS_MAX_TAPS); ++d) {
                     if (NTAP || d < tp.nd) {
-                        const float2 q = base[tp.dl[d]];
+                        const float2 q = base[dl(d)];
                         ub = fmaf(tp.w[d], q.x, ub);
                         vb = fmaf(tp.w[d], q.y, vb);
                     }
@@ -421,7 +471,17 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
                     uvw[tid + j * NT] = q;
                     if (CLIP) hs_ghost_store<PITCH>(uvw, tid + j * NT, pkj, q);
                 }
-                if (LOUT) {
+                if (LREG && lo[j] != ~0u) {
+                    const int64_t so = (int64_t)(it - 1) * a.lstride + lcorner;
+                    auto at = [&](float *plane) {
+                        return reinterpret_cast<float *>(reinterpret_cast<char *>(plane + so) +
+                                                         lo[j]);
+                    };
+                    if (a.lt) st_out(at(a.lt), tt, lwt);  // (t is optional)
+                    st_out(at(a.lub), ub, lwt);
+                    st_out(at(a.lvb), vb, lwt);
+                }
+                if (LOUT && !LREG) {
                     const int rx = pkj & 0xfff, ry = (pkj >> 16) & 0x7ff;
                     const int lx = gx0 + rx - x0, ly = gy0 + ry - y0;
                     if (lx >= 0 && lx < TW && ly >= 0 && ly < TH) {
@@ -449,7 +509,7 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
                 if (CLIP) hs_ghost_store<PITCH>(uv, tid + j * NT, pk[j], q);
             }
         }
-        if (LOUT) {
+        if (LOUT && !LREG) {
             // lin is rewritten only in the next step's compute, after the
             // barrier below
             const int64_t so = (int64_t)(it - 1) * a.lstride;
@@ -541,21 +601,12 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
 // pitch 80 (160 dwords, 32 mod the 64 LDS banks) every lane of such a read
 // fell on the same two bank pairs.  At 162 dwords the 16 lanes served in one
 // LDS cycle touch 16 distinct bank pairs.
-constexpr int HS_RING_OX[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
-constexpr int HS_RING_OY[8] = {0, 0, -1, 1, -1, -1, 1, 1};
 constexpr int HS_RING_PITCH = 64 + 2 * 8 + 1;
 constexpr int HS_RING_NPX = HS_RING_PITCH * (32 + 2 * 8);
 // LDS of the ring form's two windows, with PITCH + 1 cells before and after
 // them for the taps of spare lanes (the kernel passes the allocation + PAD)
 constexpr int HS_RING_PAD = HS_RING_PITCH + 1;
 constexpr int HS_RING_LDS = HS_RING_PAD + 2 * HS_RING_NPX + HS_RING_PAD;
-
-__host__ __device__ inline bool hs_ring_taps(const Taps &t) {
-    if (t.nd != 8) return false;
-    for (int d = 0; d < 8; ++d)
-        if (t.ox[d] != HS_RING_OX[d] || t.oy[d] != HS_RING_OY[d]) return false;
-    return true;
-}
 
 template <bool FDIV>
 __device__ __forceinline__ void hs_tile_ring(const HsImage &a, int t, int k,

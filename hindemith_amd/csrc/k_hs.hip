@@ -49,12 +49,14 @@ struct HsArgs {
 // The linearisation-output form (the adjoint's segment recompute) is compiled
 // for 8 waves per SIMD so its extra stores do not cost a work-group per CU.
 // DB: two LDS windows, one barrier per step (see hs_tile).
+// RING: the canonical 8 taps at compile-time LDS offsets (hs_tile); with LOUT
+// and DB, the one-barrier step storing the linearisation from registers.
 template <int NT, int TW, int TH, int KMAX, int NTAP, bool LOUT = false, bool DB = false,
-          bool FDIV = false>
+          bool FDIV = false, bool RING = false>
 __global__ __launch_bounds__(NT, LOUT ? 8 : 1) void hs_tb_kernel(HsArgs a, Taps taps) {
     using Win = HsWindow<NT, TW, TH, KMAX, DB>;
     __shared__ float2 uvm[Win::SIZE];
-    __shared__ float lin[LOUT ? 3 * TW * TH : 1];
+    __shared__ float lin[LOUT && !DB ? 3 * TW * TH : 1];
     const int64_t off = (int64_t)blockIdx.y * a.bstride;  // pair of a batch
     HsImage im;
     im.Ix = a.Ix + off;
@@ -76,8 +78,14 @@ __global__ __launch_bounds__(NT, LOUT ? 8 : 1) void hs_tb_kernel(HsArgs a, Taps 
         im.lstride = a.lstride;
     }
     const HsTapTable<TW + 2 * KMAX> tp(taps, NTAP);
-    hs_tile<NT, TW, TH, KMAX, NTAP, LOUT, DB, FDIV>(im, xcd_remap(blockIdx.x, a.ntiles, a.rev), a.k,
-                                                    tp, uvm + Win::PAD, LOUT ? lin : nullptr);
+    const int t = xcd_remap(blockIdx.x, a.ntiles, a.rev);
+    if constexpr (RING)
+        // shared tap addresses cost no registers here: one base plus constants
+        hs_tile<NT, TW, TH, KMAX, NTAP, LOUT, DB, FDIV, true, DB && !LOUT, false, true>(
+            im, t, a.k, tp, uvm + Win::PAD, nullptr);
+    else
+        hs_tile<NT, TW, TH, KMAX, NTAP, LOUT, DB, FDIV>(im, t, a.k, tp, uvm + Win::PAD,
+                                                        LOUT ? lin : nullptr);
 }
 
 // Ring-ordered form (hs_tile_ring): 64 x 32 tiles, k <= 8, double-buffered,
@@ -390,6 +398,11 @@ int launch_hs_tb(int variant, const float *Ix, const float *Iy, const float *It,
             return fail(HMX_ERR_UNSUPPORTED,
                         "hs_tb: linearisation output needs the 1024x64x32x8 variant");
         kern = kLinKernels[tap_class(taps.nd)];
+        // the canonical stencil: one barrier per step, the linearisation
+        // stored from registers (HMX_LIN_STAGED=1 keeps the staged form)
+        const char *staged = std::getenv("HMX_LIN_STAGED");
+        if (hs_ring_taps(taps) && !(staged && staged[0] == '1'))
+            kern = hs_tb_kernel<1024, 64, 32, 8, 8, true, true, true, true>;
     }
     hipLaunchKernelGGL(kern, dim3(a.ntiles, B), dim3(V.nt), 0, s, a, taps);
     HMX_LAUNCH_CHECK("hs_tb");

@@ -119,6 +119,22 @@ def test_ring_build_matches_generic_build_bitwise(shape, monkeypatch):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("shape", [(100, 200), (37, 66)])
+def test_register_linearisation_matches_staged_bitwise(shape, monkeypatch):
+    """The segment recompute's one-barrier form (each step's (ubar, vbar)
+    stored from registers) and its two-barrier form (staged through LDS, stored
+    in tile order) recompute the same states and store the same values:
+    identical gradients.  21 steps in segments of 10 cover recompute launches
+    of 8, 2 and 1 steps."""
+    H, W = shape
+    I1, I2, fx, fy, alpha = batch(2, H, W, 7)
+    reg = run_grad(I1, I2, fx, fy, alpha, 21, 10)
+    monkeypatch.setenv("HMX_LIN_STAGED", "1")
+    staged = run_grad(I1, I2, fx, fy, alpha, 21, 10)
+    for a, b in zip(reg, staged):
+        assert np.array_equal(a, b)
+
+
 @pytest.fixture(scope="module")
 def config5():
     """8 pairs at 512x512, 200 iterations, checkpoints every 20."""

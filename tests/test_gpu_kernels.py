@@ -99,6 +99,31 @@ def test_csr_spmv(K, lanes):
     assert np.all(np.abs(y - ref) <= bound)
 
 
+@pytest.mark.parametrize("M", [1, 63, 64, 1000, 4099])
+def test_csr_one_lane_staged_and_direct_bitexact(K, M):
+    """One lane per row: a wavefront stages its 64 rows' entries through LDS
+    when they number at most 640, else runs the direct loop.  Short rows (0-8,
+    some empty), with a few 300-entry rows that push their wavefronts past the
+    cap, and row counts that end mid-wavefront: both paths in one launch, the
+    reference's sequential fma chain bit for bit."""
+    rng = np.random.default_rng(M)
+    N = 3000
+    lens = rng.integers(0, 9, M)
+    lens[rng.random(M) < 0.2] = 0
+    lens[rng.random(M) < 0.01] = 300
+    indptr = np.zeros(M + 1, np.int32)
+    indptr[1:] = np.cumsum(lens)
+    nnz = int(indptr[-1])
+    indices = rng.integers(0, N, nnz).astype(np.int32)
+    data = rng.normal(size=nnz).astype(np.float32)
+    x = rng.uniform(-1, 1, N).astype(np.float32)
+    spans = [int(indptr[min(w + 64, M)] - indptr[w]) for w in range(0, M, 64)]
+    if M >= 1000:
+        assert min(spans) <= 640 < max(spans)  # both paths are taken
+    y = host(K.csr_spmv(dev(indptr), dev(indices), dev(data), dev(x), lanes_per_row=1))
+    assert np.array_equal(y, oracle.csr_spmv(indptr, indices, data, x))
+
+
 def test_csr_binned_matches_vector(K):
     rng = np.random.default_rng(3)
     M, N = 5000, 5000

@@ -611,7 +611,7 @@ constexpr int HS_RING_LDS = HS_RING_PAD + 2 * HS_RING_NPX + HS_RING_PAD;
 template <bool FDIV>
 __device__ __forceinline__ void hs_tile_ring(const HsImage &a, int t, int k,
                                              const HsTapTable<HS_RING_PITCH> &tp, float2 *uv) {
-    constexpr int NT = 1024, TW = 64, TH = 32, KMAX = 8;
+    constexpr int NT = 1024, TW = 64, TH = 32;  // k <= 8 (the window's ring width)
     constexpr int PITCH = HS_RING_PITCH, NPX = HS_RING_NPX;
     constexpr int P = 4;  // cell slots per thread: 32 core rows + 4 per ring, <= 64 slots
     static_assert(NPX <= P * NT, "window larger than the loader pass");

@@ -499,7 +499,7 @@ __global__ __launch_bounds__(256) void galpha_partial_kernel(const float *__rest
     for (int64_t i = lo + threadIdx.x; i < hi; i += 256) s += (double)g[i];
     part[threadIdx.x] = s;
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
+    for (unsigned w = 128; w > 0; w >>= 1) {
         if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
         __syncthreads();
     }

@@ -76,9 +76,7 @@ __global__ __launch_bounds__(256) void stencil2d_cols_kernel(const float *__rest
     float acc[ST_V];
 #pragma unroll
     for (int v = 0; v < ST_V; ++v) acc[v] = 0.0f;
-#pragma unroll
-    for (int d = 0; d < (NTAP ? NTAP : HMX_MAX_TAPS); ++d) {
-        if (!NTAP && d >= taps.nd) break;
+    auto tap = [&](int d) {
         const int xx = clampi(c + taps.ox[d], 0, W - 1);
         const float w = taps.w[d];
 #pragma unroll
@@ -86,6 +84,12 @@ __global__ __launch_bounds__(256) void stencil2d_cols_kernel(const float *__rest
             const int yy = clampi(r0 + v + taps.oy[d], 0, H - 1);
             acc[v] = fmaf(w, x[(int64_t)yy * W + xx], acc[v]);
         }
+    };
+    if (NTAP > 0) {
+#pragma unroll
+        for (int d = 0; d < (NTAP > 0 ? NTAP : 1); ++d) tap(d);
+    } else {
+        for (int d = 0; d < taps.nd; ++d) tap(d);
     }
 #pragma unroll
     for (int v = 0; v < ST_V; ++v)

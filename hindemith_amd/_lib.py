@@ -82,9 +82,9 @@ _SIGS = {
     "hmx_hs_batched_release": [],
     "hmx_hs_ckpt_count": [c_int, c_int],
     "hmx_hs_forward_ckpt": [vp, vp, c_int, c_int, c_int, vp] + [c_int, vp, vp, vp] * 4 +
-                           [c_int, c_int, vp, vp, vp, vp, vp],
+                           [c_int, c_int, vp, vp, vp, vp, vp, vp],
     "hmx_hs_adjoint": [vp, vp, c_int, c_int, c_int, vp] + [c_int, vp, vp, vp] * 4 +
-                      [c_int, c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp],
+                      [c_int, c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp],
     "hmx_hs_solve_f64": [vp, vp, c_int, c_int, c_int, vp] + [c_int, vp, vp, vp] * 4 +
                         [c_int, vp, vp, vp, vp],
 }

@@ -6,7 +6,9 @@ frame stacks and per-pair ``alpha`` (B,) on the GPU and returns the flow
 and keeps the state every ``ckpt_every`` iterations; the backward pass
 (hand-written adjoint kernel, libhmx ``hmx_hs_adjoint``) recomputes one
 segment at a time and walks it backwards, so memory is
-(iters/ckpt_every + ckpt_every) state pairs instead of iters.
+(iters/ckpt_every + ckpt_every) state pairs instead of iters.  The forward
+keeps the last segment's linearisation (up to ckpt_every more pairs of
+planes), so the backward recomputes every segment but that one.
 Gradients flow to I1, I2, alpha and, when given, the initial flow u0, v0.
 """
 from __future__ import annotations
@@ -50,17 +52,29 @@ class HornSchunckFunction(torch.autograd.Function):
         ws = torch.empty(int(L.hmx_hs_forward_workspace(B, H, W)) // 4 + 1, device=I1.device,
                          dtype=torch.float32)
         keep, targs = _tap_args(taps)
+        # With a gradient to come, the forward also keeps its last checkpoint
+        # segment's linearisation (the backward then recomputes one segment
+        # fewer) and its derivative planes (in ws); the gradients are the same
+        # bits as without them.
+        want = any(ctx.needs_input_grad[:5]) and int(iters) > 0
+        last = int(iters) - (nck - 1) * int(ckpt_every) if want else 0
+        lin = (torch.empty((2 * last, B, H, W), device=I1.device, dtype=torch.float32)
+               if want else None)
         check(L.hmx_hs_forward_ckpt(_ptr(I1), _ptr(I2), B, H, W, _ptr(alpha), *targs,
                                     int(iters), int(ckpt_every), _ptr(u), _ptr(v), _ptr(ckpt),
+                                    _ptr(lin) if lin is not None else ctypes.c_void_p(0),
                                     _ptr(ws), stream_ptr()), "hs_forward_ckpt")
-        ctx.save_for_backward(I1, I2, alpha, ckpt)
+        if lin is None:
+            lin = torch.empty(0, device=I1.device)
+        ctx.save_for_backward(I1, I2, alpha, ckpt, lin, ws)
         ctx.iters, ctx.ckpt_every, ctx.taps = int(iters), int(ckpt_every), taps
         ctx.has_u0 = u0 is not None
+        ctx.kept = want
         return u, v
 
     @staticmethod
     def backward(ctx, gu, gv):
-        I1, I2, alpha, ckpt = ctx.saved_tensors
+        I1, I2, alpha, ckpt, lin, fws = ctx.saved_tensors
         B, H, W = I1.shape
         gu = torch.zeros_like(I1) if gu is None else gu.contiguous()
         gv = torch.zeros_like(I1) if gv is None else gv.contiguous()
@@ -75,7 +89,8 @@ class HornSchunckFunction(torch.autograd.Function):
         keep, targs = _tap_args(ctx.taps)
         null = ctypes.c_void_p(0)
         check(L.hmx_hs_adjoint(_ptr(I1), _ptr(I2), B, H, W, _ptr(alpha), *targs, ctx.iters,
-                               ctx.ckpt_every, _ptr(ckpt), _ptr(gu), _ptr(gv), _ptr(gI1),
+                               ctx.ckpt_every, _ptr(ckpt), _ptr(lin) if ctx.kept else null,
+                               _ptr(fws) if ctx.kept else null, _ptr(gu), _ptr(gv), _ptr(gI1),
                                _ptr(gI2), _ptr(ga), _ptr(gu0) if gu0 is not None else null,
                                _ptr(gv0) if gv0 is not None else null, _ptr(ws), stream_ptr()),
               "hs_adjoint")

@@ -560,6 +560,40 @@ static int64_t adj_tiles(int H, int W) {
 constexpr int ADJ_RECOMPUTE_VARIANT = 3;  // 1024 threads, 64x32 tiles, k <= 8
 constexpr int ADJ_RECOMPUTE_K = 8;
 
+int launch_hs_copy2(const float *u, const float *v, float *wu, float *wv, int64_t n,
+                    hipStream_t s);
+
+// len iterations from the state in (u, v) with the linearisation-output
+// launches (ADJ_RECOMPUTE_K per launch), storing iteration j's (ubar, vbar) at
+// lin + 2*j*n and + n, the state ending in (u, v): the launches ping-pong
+// through (wu, wv), the first reading (wu, wv) after a copy when their count
+// is odd.  The same launches, in the same order, as the adjoint's segment
+// recompute, so the states and the stored values are the same bits.
+static int lin_segment(const float *Ix, const float *Iy, const float *It, int H, int W,
+                       const Taps &av, int len, float *u, float *v, float *wu, float *wv,
+                       float *lin, int64_t n, hipStream_t s, int B, const float *alpha2) {
+    const int nl = (len + ADJ_RECOMPUTE_K - 1) / ADJ_RECOMPUTE_K;
+    float *su = u, *sv = v, *du = wu, *dv = wv;
+    if (nl & 1) {
+        int rc = launch_hs_copy2(u, v, wu, wv, n, s);
+        if (rc) return rc;
+        std::swap(su, du);
+        std::swap(sv, dv);
+    }
+    for (int j0 = 0; j0 < len;) {
+        const int kk = std::min(ADJ_RECOMPUTE_K, len - j0);
+        float *lub = lin + (int64_t)2 * j0 * n;
+        int rc = launch_hs_tb(ADJ_RECOMPUTE_VARIANT, Ix, Iy, It, nullptr, 0.0f, H, W, av, kk, su,
+                              sv, du, dv, s, B, alpha2, nullptr, lub, lub + n, 2 * n,
+                              (j0 / ADJ_RECOMPUTE_K) & 1);
+        if (rc) return rc;
+        std::swap(su, du);
+        std::swap(sv, dv);
+        j0 += kk;
+    }
+    return HMX_OK;
+}
+
 }  // namespace hmx
 
 using namespace hmx;
@@ -588,8 +622,8 @@ extern "C" int hmx_hs_forward_ckpt(const float *I1, const float *I2, int B, int 
                                    const int *oxy, const int *oyy, int ndt, const float *wt,
                                    const int *oxt, const int *oyt, int nda, const float *wa,
                                    const int *oxa, const int *oya, int iters, int ckpt_every,
-                                   float *u, float *v, float *ckpt, void *workspace,
-                                   void *stream) {
+                                   float *u, float *v, float *ckpt, float *lin_last,
+                                   void *workspace, void *stream) {
     HMX_REQUIRE(I1 && I2 && alpha && u && v && workspace, "hmx_hs_forward_ckpt: null argument");
     HMX_REQUIRE(B >= 1 && H > 0 && W > 0 && iters >= 0 && ckpt_every >= 1,
                 "hmx_hs_forward_ckpt: bad sizes");
@@ -616,8 +650,13 @@ extern "C" int hmx_hs_forward_ckpt(const float *I1, const float *I2, int B, int 
         HMX_HIP_TRY(hipMemcpyAsync(cu, u, n * 4, hipMemcpyDeviceToDevice, s));
         HMX_HIP_TRY(hipMemcpyAsync(cv, v, n * 4, hipMemcpyDeviceToDevice, s));
         const int len = (iters - c * ckpt_every) < ckpt_every ? iters - c * ckpt_every : ckpt_every;
-        rc = hs_jacobi_run(Ix, Iy, It, nullptr, 0.0f, H, W, T.av, len, u, v, wu, wv, variant, k, s,
-                           B, alpha2, false);
+        if (lin_last && c == nck - 1)
+            // the last segment also keeps each iteration's (ubar, vbar): the
+            // reverse pass then needs no recompute of it
+            rc = lin_segment(Ix, Iy, It, H, W, T.av, len, u, v, wu, wv, lin_last, n, s, B, alpha2);
+        else
+            rc = hs_jacobi_run(Ix, Iy, It, nullptr, 0.0f, H, W, T.av, len, u, v, wu, wv, variant,
+                               k, s, B, alpha2, false);
         if (rc) return rc;
     }
     return HMX_OK;
@@ -629,6 +668,7 @@ extern "C" int hmx_hs_adjoint(const float *I1, const float *I2, int B, int H, in
                               const int *oyy, int ndt, const float *wt, const int *oxt,
                               const int *oyt, int nda, const float *wa, const int *oxa,
                               const int *oya, int iters, int ckpt_every, const float *ckpt,
+                              const float *lin_last, const void *fwd_workspace,
                               const float *gu_N, const float *gv_N, float *gI1, float *gI2,
                               float *galpha, float *gu0, float *gv0, void *workspace,
                               void *stream) {
@@ -647,6 +687,12 @@ extern "C" int hmx_hs_adjoint(const float *I1, const float *I2, int B, int H, in
     HMX_REQUIRE(ntiles < (1 << 30), "hmx_hs_adjoint: image too large");
     float *ws = (float *)workspace;
     float *Ix = ws, *Iy = ws + n, *It = ws + 2 * n;
+    if (fwd_workspace) {  // the forward's derivative planes (its workspace's first three)
+        float *f = (float *)const_cast<void *>(fwd_workspace);
+        Ix = f;
+        Iy = f + n;
+        It = f + 2 * n;
+    }
     float *gIx = ws + 3 * n, *gIy = ws + 4 * n, *gIt = ws + 5 * n;
     float *g[2][2] = {{ws + 6 * n, ws + 7 * n}, {ws + 8 * n, ws + 9 * n}};
     float *rs[2][2] = {{ws + 10 * n, ws + 11 * n}, {ws + 12 * n, ws + 13 * n}};
@@ -657,13 +703,14 @@ extern "C" int hmx_hs_adjoint(const float *I1, const float *I2, int B, int H, in
 
     hipLaunchKernelGGL(alpha_sq_kernel, dim3((B + 63) / 64), dim3(64), 0, s, alpha, alpha2, B);
     HMX_LAUNCH_CHECK("alpha_sq");
-    rc = setup_batch(I1, I2, B, H, W, T.dx, T.dy, T.dt, Ix, Iy, It, s);
+    rc = fwd_workspace ? HMX_OK : setup_batch(I1, I2, B, H, W, T.dx, T.dy, T.dt, Ix, Iy, It, s);
     if (!rc) rc = launch_fill_f32(gIx, 0.0f, 3 * n, s);
     if (!rc) rc = launch_fill_f32(ga2, 0.0f, n, s);
     if (rc) return rc;
-    HMX_HIP_TRY(hipMemcpyAsync(g[0][0], gu_N, n * 4, hipMemcpyDeviceToDevice, s));
-    HMX_HIP_TRY(hipMemcpyAsync(g[0][1], gv_N, n * 4, hipMemcpyDeviceToDevice, s));
-    int cur = 0;
+    // the gradient in: dL/dx_N itself for the first reverse launch, then the
+    // ping-pong pair g[0], g[1] (the launch writing g[cur ^ 1] reads the other)
+    const float *gin_u = gu_N, *gin_v = gv_N;
+    int cur = 1;
 
     AdjTbArgs a;
     std::memset(&a, 0, sizeof(a));
@@ -692,8 +739,11 @@ extern "C" int hmx_hs_adjoint(const float *I1, const float *I2, int B, int H, in
         // launch, keeping each step's (ubar, vbar); the states themselves
         // are bit-identical to the forward pass and only ping-pong here
         const float *cu = ckpt + (int64_t)(2 * c) * n, *cv = ckpt + (int64_t)(2 * c + 1) * n;
+        // the last segment's (ubar, vbar) may come from the forward pass
+        const bool kept = lin_last && c == nck - 1;
+        float *seg_lin = kept ? const_cast<float *>(lin_last) : lin;
         int rcur = 0;
-        for (int j0 = 0; j0 < len;) {
+        for (int j0 = 0; j0 < len && !kept;) {
             const int kk = std::min(ADJ_RECOMPUTE_K, len - j0);
             float *lub = lin + (int64_t)2 * j0 * n;
             rc = launch_hs_tb(ADJ_RECOMPUTE_VARIANT, Ix, Iy, It, nullptr, 0.0f, H, W, T.av, kk,
@@ -708,13 +758,13 @@ extern "C" int hmx_hs_adjoint(const float *I1, const float *I2, int B, int H, in
         for (int hi = len; hi > 0;) {
             const int mm = std::min(ADJ_MMAX, hi);
             a.m = mm;
-            a.lin_hi = lin + (int64_t)2 * (hi - 1) * n;
+            a.lin_hi = seg_lin + (int64_t)2 * (hi - 1) * n;
             a.lin_step = (int64_t)2 * n;
             a.plane = n;
-            a.gu_in = g[cur][0];
-            a.gv_in = g[cur][1];
-            a.gu_out = g[cur ^ 1][0];
-            a.gv_out = g[cur ^ 1][1];
+            a.gu_in = gin_u;
+            a.gv_in = gin_v;
+            a.gu_out = g[cur][0];
+            a.gv_out = g[cur][1];
             const dim3 grid((unsigned)ntiles, B);
             if (ring)
                 hipLaunchKernelGGL((hs_adj_tb_kernel<ADJ_NT, ADJ_TW, ADJ_TH, ADJ_MMAX, 8, true>),
@@ -726,12 +776,15 @@ extern "C" int hmx_hs_adjoint(const float *I1, const float *I2, int B, int H, in
                 hipLaunchKernelGGL((hs_adj_tb_kernel<ADJ_NT, ADJ_TW, ADJ_TH, ADJ_MMAX, 0, false>),
                                    grid, dim3(ADJ_NT), 0, s, a, T.av);
             HMX_LAUNCH_CHECK("hs_adj_tb");
+            gin_u = g[cur][0];
+            gin_v = g[cur][1];
             cur ^= 1;
             hi -= mm;
         }
     }
-    if (gu0) HMX_HIP_TRY(hipMemcpyAsync(gu0, g[cur][0], n * 4, hipMemcpyDeviceToDevice, s));
-    if (gv0) HMX_HIP_TRY(hipMemcpyAsync(gv0, g[cur][1], n * 4, hipMemcpyDeviceToDevice, s));
+    // dL/dx_0: the last launch's output (dL/dx_N itself when iters == 0)
+    if (gu0) HMX_HIP_TRY(hipMemcpyAsync(gu0, gin_u, n * 4, hipMemcpyDeviceToDevice, s));
+    if (gv0) HMX_HIP_TRY(hipMemcpyAsync(gv0, gin_v, n * 4, hipMemcpyDeviceToDevice, s));
     dim3 grid((W + 255) / 256, H < 65535 ? H : 65535, B);
     hipLaunchKernelGGL(hs_setup_T_kernel, grid, dim3(256), 0, s, gIx, gIy, gIt, H, W,
                        (int64_t)H * W, T.dx, T.dy, T.dt, gI1, gI2);

@@ -285,7 +285,13 @@ int64_t hmx_hs_adjoint_workspace(int B, int H, int W, int ckpt_every);
 
 /* Forward solve that stores the state (u, v) every ckpt_every iterations:
  * ckpt holds hmx_hs_ckpt_count() x 2 x B x H x W floats (state before
- * iterations 0, C, 2C, ...).  u, v: initial state in, final state out. */
+ * iterations 0, C, 2C, ...).  u, v: initial state in, final state out.
+ * lin_last (may be null): receives the last checkpoint segment's
+ * linearisation, 2 x L x B x H x W floats for its L iterations (iteration j's
+ * ubar plane at 2j, vbar at 2j + 1), which lets hmx_hs_adjoint skip that
+ * segment's recompute.  workspace keeps the derivative planes Ix, Iy, It (its
+ * first 3 x B x H x W floats), which hmx_hs_adjoint may read instead of
+ * recomputing them. */
 int hmx_hs_forward_ckpt(const float *I1, const float *I2, int B, int H, int W,
                         const float *alpha,
                         int ndx, const float *wx, const int *oxx, const int *oyx,
@@ -293,19 +299,23 @@ int hmx_hs_forward_ckpt(const float *I1, const float *I2, int B, int H, int W,
                         int ndt, const float *wt, const int *oxt, const int *oyt,
                         int nda, const float *wa, const int *oxa, const int *oya,
                         int iters, int ckpt_every, float *u, float *v, float *ckpt,
-                        void *workspace, void *stream);
+                        float *lin_last, void *workspace, void *stream);
 
 /* Reverse pass: given dL/du_N, dL/dv_N, writes dL/dI1, dL/dI2 (B x H x W),
  * dL/dalpha (B, device) and, if non-null, dL/du_0, dL/dv_0.  Recomputes one
- * checkpoint segment at a time; averaging stencil within radius 1. */
+ * checkpoint segment at a time; averaging stencil within radius 1.  Optional
+ * (null: recomputed here), from the hmx_hs_forward_ckpt call that wrote ckpt:
+ * lin_last, its last segment's linearisation, and fwd_workspace, its
+ * workspace (derivative planes).  The gradients are the same bits either way. */
 int hmx_hs_adjoint(const float *I1, const float *I2, int B, int H, int W, const float *alpha,
                    int ndx, const float *wx, const int *oxx, const int *oyx,
                    int ndy, const float *wy, const int *oxy, const int *oyy,
                    int ndt, const float *wt, const int *oxt, const int *oyt,
                    int nda, const float *wa, const int *oxa, const int *oya,
-                   int iters, int ckpt_every, const float *ckpt, const float *gu_N,
-                   const float *gv_N, float *gI1, float *gI2, float *galpha, float *gu0,
-                   float *gv0, void *workspace, void *stream);
+                   int iters, int ckpt_every, const float *ckpt, const float *lin_last,
+                   const void *fwd_workspace, const float *gu_N, const float *gv_N,
+                   float *gI1, float *gI2, float *galpha, float *gu0, float *gv0,
+                   void *workspace, void *stream);
 
 /* fp64 forward solve (plain per-iteration kernels) of B pairs, for
  * finite-difference checks.  alpha: device doubles; workspace 5*B*H*W doubles. */

@@ -135,6 +135,51 @@ def test_register_linearisation_matches_staged_bitwise(shape, monkeypatch):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("shape,iters,ckpt", [((100, 200), 37, 10), ((37, 66), 21, 7),
+                                              ((40, 52), 8, 8)])
+def test_kept_linearisation_and_planes_match_recompute_bitwise(shape, iters, ckpt):
+    """The backward as autograd runs it (the forward's last-segment
+    linearisation and derivative planes reused) against the C ABI called with
+    neither (every segment recomputed, the planes rebuilt): the same gradients
+    bit for bit, for last segments of 7, 7 and 8 iterations."""
+    import ctypes
+    from hindemith_amd import autograd as AG
+    from hindemith_amd._lib import check, lib
+    from hindemith_amd.kernels import _ptr, stream_ptr
+    H, W = shape
+    B = 2
+    I1, I2, fx, fy, alpha = batch(B, H, W, 8)
+    d = "cuda"
+    tI1 = torch.tensor(I1, device=d, requires_grad=True)
+    tI2 = torch.tensor(I2, device=d, requires_grad=True)
+    ta = torch.tensor(alpha, device=d, requires_grad=True)
+    u, v = AG.horn_schunck(tI1, tI2, ta, iters=iters, ckpt_every=ckpt)
+    rng = np.random.default_rng(1)
+    gu = torch.tensor(rng.normal(size=(B, H, W)).astype(np.float32), device=d)
+    gv = torch.tensor(rng.normal(size=(B, H, W)).astype(np.float32), device=d)
+    g1, g2, ga = torch.autograd.grad((u, v), (tI1, tI2, ta), (gu, gv))
+
+    L = lib()
+    keep, targs = AG._tap_args(AG.default_taps())
+    null = ctypes.c_void_p(0)
+    dI1, dI2, dal = (t.detach().contiguous() for t in (tI1, tI2, ta))
+    uu, vv = torch.zeros_like(dI1), torch.zeros_like(dI1)
+    nck = L.hmx_hs_ckpt_count(iters, ckpt)
+    ck = torch.empty((nck, 2, B, H, W), device=d)
+    fws = torch.empty(int(L.hmx_hs_forward_workspace(B, H, W)) // 4 + 1, device=d)
+    check(L.hmx_hs_forward_ckpt(_ptr(dI1), _ptr(dI2), B, H, W, _ptr(dal), *targs, iters, ckpt,
+                                _ptr(uu), _ptr(vv), _ptr(ck), null, _ptr(fws), stream_ptr()),
+          "forward")
+    assert torch.equal(uu, u.detach()) and torch.equal(vv, v.detach())
+    r1, r2, rga = torch.empty_like(dI1), torch.empty_like(dI1), torch.empty_like(dal)
+    ws = torch.empty(int(L.hmx_hs_adjoint_workspace(B, H, W, ckpt)) // 4 + 1, device=d)
+    check(L.hmx_hs_adjoint(_ptr(dI1), _ptr(dI2), B, H, W, _ptr(dal), *targs, iters, ckpt,
+                           _ptr(ck), null, null, _ptr(gu), _ptr(gv), _ptr(r1), _ptr(r2),
+                           _ptr(rga), null, null, _ptr(ws), stream_ptr()), "adjoint")
+    torch.cuda.synchronize()
+    assert torch.equal(g1, r1) and torch.equal(g2, r2) and torch.equal(ga, rga)
+
+
 @pytest.fixture(scope="module")
 def config5():
     """8 pairs at 512x512, 200 iterations, checkpoints every 20."""

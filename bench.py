@@ -286,8 +286,9 @@ def extra_batched(S):
                   [p["alpha"] for p in pairs], iters=200)
     vi, k = B.tune()
     B.run(graph=True, sync=True)
-    ms_b = _time_ms(lambda: B.run(graph=True), 5)
-    ms_p = _time_ms(B.solve_each, 3, warm=1)
+    # both sides timed over ~0.1-0.2 s after warm-up calls, at the sustained rate
+    ms_b = _time_ms(lambda: B.run(graph=True), 20, warm=5)
+    ms_p = _time_ms(B.solve_each, 6, warm=2)
     # the same 64 solves captured in one hipGraph: GPU time without host cost
     ms_pg = _time_graph_ms(B.solve_each, launches=1, reps=3)
     ms_1 = _time_ms(lambda: B.solve_each(variant=0, k=1), 2, warm=1)

@@ -173,9 +173,9 @@ struct HsWindow {
 template <bool COH>
 __device__ __forceinline__ float hs_ld_uv(const char *p) {
     if (COH)
-        return __hip_atomic_load(reinterpret_cast<const float *>(p), __ATOMIC_RELAXED,
+        return __hip_atomic_load((const gfloat *)p, __ATOMIC_RELAXED,
                                  __HIP_MEMORY_SCOPE_SYSTEM);
-    return *reinterpret_cast<const float *>(p);
+    return ld_g(reinterpret_cast<const float *>(p));
 }
 // The output tile is stored through the L2 (st_out) when the image's rows
 // fill whole cache lines, as every kernel's output here (1080p launches 31.9
@@ -274,12 +274,12 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
             // registers are left unset
             if (in[j]) {
                 const uint64_t o = (unsigned)ry * rowb + (unsigned)rx * (unsigned)sizeof(float);
-                cx[j] = *reinterpret_cast<const float *>(pIx + o);
-                cy[j] = *reinterpret_cast<const float *>(pIy + o);
-                ct[j] = *reinterpret_cast<const float *>(pIt + o);
+                cx[j] = ld_g(reinterpret_cast<const float *>(pIx + o));
+                cy[j] = ld_g(reinterpret_cast<const float *>(pIy + o));
+                ct[j] = ld_g(reinterpret_cast<const float *>(pIt + o));
                 ui[j] = hs_ld_uv<COH>(pu + o);
                 vi[j] = hs_ld_uv<COH>(pv + o);
-                if (pd) cd[j] = *reinterpret_cast<const float *>(pd + o);
+                if (pd) cd[j] = ld_g(reinterpret_cast<const float *>(pd + o));
             }
         }
 #pragma unroll
@@ -339,10 +339,10 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
                 vi[j] = hs_ld_uv<COH>(bv + o);
             }
             if (img[j]) {
-                cx[j] = *reinterpret_cast<const float *>(bIx + o);
-                cy[j] = *reinterpret_cast<const float *>(bIy + o);
-                ct[j] = *reinterpret_cast<const float *>(bIt + o);
-                if (bd) cd[j] = *reinterpret_cast<const float *>(bd + o);
+                cx[j] = ld_g(reinterpret_cast<const float *>(bIx + o));
+                cy[j] = ld_g(reinterpret_cast<const float *>(bIy + o));
+                ct[j] = ld_g(reinterpret_cast<const float *>(bIt + o));
+                if (bd) cd[j] = ld_g(reinterpret_cast<const float *>(bd + o));
             }
         }
 #pragma unroll
@@ -642,8 +642,8 @@ __device__ __forceinline__ void hs_tile_ring(const HsImage &a, int t, int k,
         if (((j + 1) * NT <= NPX || c < NPX) && rx < RW && ry < RH) {
             const unsigned o = (unsigned)clampi(gy0 + ry, 0, a.H - 1) * rowb +
                                (unsigned)clampi(gx0 + rx, 0, a.W - 1) * (unsigned)sizeof(float);
-            uv[c] = make_float2(*reinterpret_cast<const float *>(bu + o),
-                                *reinterpret_cast<const float *>(bv + o));
+            uv[c] = make_float2(ld_g(reinterpret_cast<const float *>(bu + o)),
+                                ld_g(reinterpret_cast<const float *>(bv + o)));
         }
     }
 
@@ -684,10 +684,10 @@ __device__ __forceinline__ void hs_tile_ring(const HsImage &a, int t, int k,
                                          (gy == a.H - 1) << 3;
             pk[j] = (L << 27) | (e << 12);
             const unsigned o = (unsigned)gy * rowb + (unsigned)gx * (unsigned)sizeof(float);
-            cx[j] = *reinterpret_cast<const float *>(bIx + o);
-            cy[j] = *reinterpret_cast<const float *>(bIy + o);
-            ct[j] = *reinterpret_cast<const float *>(bIt + o);
-            if (bd) cd[j] = *reinterpret_cast<const float *>(bd + o);
+            cx[j] = ld_g(reinterpret_cast<const float *>(bIx + o));
+            cy[j] = ld_g(reinterpret_cast<const float *>(bIy + o));
+            ct[j] = ld_g(reinterpret_cast<const float *>(bIt + o));
+            if (bd) cd[j] = ld_g(reinterpret_cast<const float *>(bd + o));
         }
     }
 #pragma unroll

@@ -106,7 +106,7 @@ __device__ inline float setup_tap(const float *__restrict__ x, int H, int W, int
     float acc = 0.0f;
     for (int d = 0; d < t.nd; ++d) {
         const int xx = clampi(c + t.ox[d], 0, W - 1), yy = clampi(r + t.oy[d], 0, H - 1);
-        acc = fmaf(t.w[d], x[(int64_t)yy * W + xx], acc);
+        acc = fmaf(t.w[d], ld_g(x + (int64_t)yy * W + xx), acc);
     }
     return acc;
 }
@@ -118,7 +118,7 @@ __device__ __forceinline__ float setup_tap4(const float *__restrict__ x, int H, 
 #pragma unroll
     for (int d = 0; d < 4; ++d) {
         const int xx = clampi(c + t.ox[d], 0, W - 1), yy = clampi(r + t.oy[d], 0, H - 1);
-        v[d] = x[(int64_t)yy * W + xx];
+        v[d] = ld_g(x + (int64_t)yy * W + xx);
     }
     float acc = 0.0f;
 #pragma unroll
@@ -151,17 +151,17 @@ __device__ __forceinline__ void setup_tile_cube(const BatchArgs &b, const SetupT
 #pragma unroll
     for (int j = 0; j <= RPT; ++j) {
         const int64_t row = (int64_t)min(r0 + j, P.H - 1) * P.W;
-        a0[j] = P.I1[row + c];
-        a1[j] = P.I1[row + c1];
-        b0[j] = P.I2[row + c];
-        b1[j] = P.I2[row + c1];
+        a0[j] = ld_g(P.I1 + row + c);
+        a1[j] = ld_g(P.I1 + row + c1);
+        b0[j] = ld_g(P.I2 + row + c);
+        b1[j] = ld_g(P.I2 + row + c1);
     }
     if (b.phase0) {  // the first phase reads the start state from (wu, wv)
 #pragma unroll
         for (int q = 0; q < RPT; ++q) {
             const int64_t p = (int64_t)min(r0 + q, P.H - 1) * P.W + c;
-            uu[q] = P.u[p];
-            vv[q] = P.v[p];
+            uu[q] = ld_g(P.u + p);
+            vv[q] = ld_g(P.v + p);
         }
     }
 #pragma unroll
@@ -198,8 +198,8 @@ __device__ __forceinline__ void setup_tile(const BatchArgs &b, const SetupTaps &
         if (r >= P.H || c >= P.W) continue;
         const int64_t p = (int64_t)r * P.W + c;
         if (b.phase0) {  // the first phase reads the start state from (wu, wv)
-            st_out(P.wu + p, P.u[p], rows_whole_lines(P.W));
-            st_out(P.wv + p, P.v[p], rows_whole_lines(P.W));
+            st_out(P.wu + p, ld_g(P.u + p), rows_whole_lines(P.W));
+            st_out(P.wv + p, ld_g(P.v + p), rows_whole_lines(P.W));
         }
         if (TAP4) {
             const float a1 = setup_tap4(P.I1, P.H, P.W, r, c, st.dx);
@@ -322,9 +322,13 @@ __global__ __launch_bounds__(NT) void hs_multi_setup_kernel(BatchArgs b, SetupTa
         setup_tile<NT, TW, TH>(b, st, w);
 }
 
+// At most 74 scalar registers: the reciprocal-quotient build compiled to 76,
+// and at 76 the hardware kept one 1024-thread work-group per CU instead of
+// two (half the waves in flight: 1.5x the cycles of the same body in
+// hs_tb_kernel, which compiles to 72); the division build (74) kept two.
 template <int NT, int TW, int TH, int KMAX, int NTAP, bool DB, bool FDIV>
-__global__ __launch_bounds__(NT) void hs_multi_phase_kernel(BatchArgs b, Taps taps, int ph,
-                                                            int kk) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_num_sgpr(74))) void hs_multi_phase_kernel(
+    BatchArgs b, Taps taps, int ph, int kk) {
     using Win = HsWindow<NT, TW, TH, KMAX, DB>;
     __shared__ float2 uvm[Win::SIZE];
     const HsTapTable<TW + 2 * KMAX> tp(taps, NTAP);

@@ -355,8 +355,11 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
                 const int dy = max(max(y0 - gy, gy - y1 + 1), 0);
                 const int e = (gx == 0) | (gx == a.W - 1) << 1 | (gy == 0) << 2 |
                               (gy == a.H - 1) << 3;
+                // an edge pixel's one ghost as a window offset in 9 signed
+                // bits (CLIP 1 below)
+                const int g = (e & 1) ? -1 : (e & 2) ? 1 : (e & 4) ? -PITCH : (e & 8) ? PITCH : 0;
                 pk[j] = ((k - max(dx, dy)) << 27) | (e << 12) |
-                        (LOUT ? (rys[j] << 16) | rxs[j] : 0);
+                        (LOUT ? (rys[j] << 16) | rxs[j] : SHARED && !COH ? (g & 0x1ff) : 0);
                 if (!bd) {
                     const float m1 = cx[j] * cx[j];
                     const float s1 = a.alpha2 + m1;
@@ -387,12 +390,23 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
     // One Jacobi step.  PAR (compile time, SHARED builds) is the window step
     // it reads with DB, (it - 1) & 1, so both windows' tap addresses are
     // constants; other builds pass 0 and select the window at run time.
-    // CLIP (compile time): the tile has ghost cells to refresh.  Unclipped
-    // tiles run a loop without the ghost stores, whose per-cell border masks
-    // would otherwise be set up for every tile.
+    // CLIP (compile time): the tile's ghost cells to refresh.  0: none (an
+    // unclipped tile: a loop without the ghost stores, whose per-cell border
+    // masks would otherwise be set up for every tile).  1 (a tile clipped
+    // along one axis): an updated pixel has at most one ghost, at the offset
+    // g that pk carries, so the refresh is one masked store.  2: any ghosts,
+    // through the border flags (hs_ghost_store: a corner pixel has three).
     auto step = [&](const int it, auto par_c, auto clip_c) {
         constexpr int PAR = decltype(par_c)::value;
-        constexpr bool CLIP = decltype(clip_c)::value;
+        constexpr int CLIP = decltype(clip_c)::value;
+        auto ghost = [&](float2 *w, int c, int pkc, float2 q) {
+            if constexpr (CLIP == 1) {
+                const int g = (pkc << 23) >> 23;
+                if (g != 0) w[c + g] = q;
+            } else if constexpr (CLIP == 2) {
+                hs_ghost_store<PITCH>(w, c, pkc, q);
+            }
+        };
         const int act_min = it << 27;
         const int par = SHARED ? PAR : (it - 1) & 1;
         const float2 *uvr = DB ? uv + par * NPX : uv;
@@ -438,7 +452,7 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
                     if (DB) {
                         const float2 q = make_float2(nu[j], nv[j]);
                         uvw[tid + j * NT] = q;
-                        if (CLIP) hs_ghost_store<PITCH>(uvw, tid + j * NT, pk[j], q);
+                        ghost(uvw, tid + j * NT, pk[j], q);
                     }
                 }
 #pragma unroll
@@ -469,7 +483,7 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
                 if (DB) {
                     const float2 q = make_float2(nu[j], nv[j]);
                     uvw[tid + j * NT] = q;
-                    if (CLIP) hs_ghost_store<PITCH>(uvw, tid + j * NT, pkj, q);
+                    ghost(uvw, tid + j * NT, pkj, q);
                 }
                 if (LREG && lo[j] != ~0u) {
                     const int64_t so = (int64_t)(it - 1) * a.lstride + lcorner;
@@ -506,7 +520,7 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
             if (act[j]) {
                 const float2 q = make_float2(nu[j], nv[j]);
                 uv[tid + j * NT] = q;
-                if (CLIP) hs_ghost_store<PITCH>(uv, tid + j * NT, pk[j], q);
+                ghost(uv, tid + j * NT, pk[j], q);
             }
         }
         if (LOUT && !LREG) {
@@ -535,10 +549,27 @@ __device__ __forceinline__ void hs_tile(const HsImage &a, int t, int k,
             for (int it = 1; it <= k; ++it) step(it, std::integral_constant<int, 0>(), clip_c);
         }
     };
-    if (clipped)
-        run(std::true_type());
-    else
-        run(std::false_type());
+    if (!clipped) {
+        run(std::integral_constant<int, 0>());
+    } else {
+        // (LOUT: pk's low bits hold the cell's window position; per-cell tap
+        // addresses and the persistent form (COH): no registers to spare for
+        // a third loop body)
+        if constexpr (LOUT || !SHARED || COH) {
+            run(std::integral_constant<int, 2>());
+        } else {
+            // An updated pixel carrying two border flags lies in a tile
+            // clipped along both axes (an image corner pixel: an updated
+            // pixel's ghost is inside the region only on a side where the
+            // region leaves the image) or in an image one pixel wide or
+            // tall; those tiles take the general form.
+            if (a.W == 1 || a.H == 1 ||
+                (((gx0 < 0) || (x1 + k > a.W)) && ((gy0 < 0) || (y1 + k > a.H))))
+                run(std::integral_constant<int, 2>());
+            else
+                run(std::integral_constant<int, 1>());
+        }
+    }
 
     // Store the output tile from the final window (complete: every step ends
     // with a barrier) in tile order: thread tid stores tile cells tid + j*NT,

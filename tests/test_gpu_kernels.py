@@ -342,11 +342,14 @@ def test_hs_jacobi_tap_classes_bitexact(K, kind):
         assert np.array_equal(host(u), ref_u) and np.array_equal(host(v), ref_v), (kind, vi, k)
 
 
-@pytest.mark.parametrize("shape", [(1, 1), (1, 70), (70, 1), (2, 3), (33, 65), (40, 129)])
+@pytest.mark.parametrize("shape", [(1, 1), (1, 70), (70, 1), (2, 3), (33, 65), (40, 129),
+                                   (100, 300), (161, 450)])
 def test_hs_jacobi_border_clamp_bitexact(K, shape):
     """Border tiles read clamped neighbours from ghost cells: a stencil with nine
     distinct weights in scrambled order tells every clamped tap apart (a ghost
-    that held the wrong pixel, or a missing corner copy, changes the sum)."""
+    that held the wrong pixel, or a missing corner copy, changes the sum).  The
+    two larger shapes have tiles clipped along one axis only (one ghost per
+    edge pixel) besides corner and interior tiles."""
     I1, I2, _, _ = S.hs_pair(*shape, seed=11)
     Ix, Iy, It = oracle.hs_setup(I1, I2, TAPS["Dx"], TAPS["Dy"], TAPS["Dt"])
     order = [4, 0, 8, 2, 6, 1, 7, 3, 5]

@@ -38,6 +38,7 @@ struct PairDesc {
     const float *I1, *I2;
     float *Ix, *Iy, *It;
     float *u, *v, *wu, *wv;
+    float *zu, *zv;  // stepped form: third plane pair (see phase_image_stepped)
     float alpha2;
     int H, W, tiles_x, tile_base, ntiles;
 };
@@ -53,7 +54,10 @@ struct BatchArgs {
     int *status;
     long long spin_limit;
     int npairs, total_tiles, iters, k;
-    int phase0;  // 1: odd phase count, phase 0 reads (wu, wv) (see phase_image)
+    // 1: the setup copies (u, v) into (wu, wv) and phase 0 reads them from
+    // there (persistent form: odd phase count; stepped: one phase)
+    int phase0;
+    int nph;  // phase count
 };
 
 struct SetupTaps {
@@ -245,6 +249,37 @@ __device__ __forceinline__ HsImage phase_image(const PairDesc &P, int ph, int ph
     return im;
 }
 
+// The stepped form's planes: phase ph reads X_ph and writes X_ph+1, where
+// X_0 = X_nph = (u, v) and, between them, X_p = (wu, wv) for odd p and
+// (zu, zv) for even p.  With an even phase count (zu, zv) is (u, v) itself;
+// with an odd one above 1 it is a plane pair of the plan's own, so no phase
+// reads what it writes and no copy is needed.  One phase (nph = 1) reads the
+// copy the setup made in (wu, wv) (phase0).
+__device__ __forceinline__ HsImage phase_image_stepped(const PairDesc &P, int ph, int nph,
+                                                       int phase0) {
+    HsImage im = phase_image(P, 0, 0);
+    auto plane = [&](int p, float *&pu, float *&pv) {
+        if (p == nph || (p == 0 && !phase0)) {
+            pu = P.u;
+            pv = P.v;
+        } else if (p & 1 || p == 0) {
+            pu = P.wu;
+            pv = P.wv;
+        } else {
+            pu = P.zu;
+            pv = P.zv;
+        }
+    };
+    float *iu, *iv, *ou, *ov;
+    plane(ph, iu, iv);
+    plane(ph + 1, ou, ov);
+    im.u_in = iu;
+    im.v_in = iv;
+    im.u_out = ou;
+    im.v_out = ov;
+    return im;
+}
+
 // WPE: waves per SIMD the kernel is compiled to allow (the compiler must fit
 // its VGPRs and SGPRs to it), from which the resident grid is derived.
 // DB: two LDS windows, one barrier per Jacobi step (see hs_tile).
@@ -337,7 +372,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_num_sgpr(74))) void hs_mu
     // data the previous launch left in that XCD's L2 and in the MALL
     const int w = xcd_remap(blockIdx.x, b.total_tiles, (ph & 1) == 0);
     const PairDesc &P = b.tiles[w];
-    const HsImage im = phase_image(P, ph, b.phase0);
+    const HsImage im = phase_image_stepped(P, ph, b.nph, b.phase0);
     hs_tile<NT, TW, TH, KMAX, NTAP, false, DB, FDIV>(im, w - P.tile_base, kk, tp,
                                                       uvm + Win::PAD);
 }
@@ -401,6 +436,7 @@ struct BatchPlan {
     int64_t sync_words = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    float *d_z = nullptr;  // stepped form, odd phase count above 1: every pair's (zu, zv)
 };
 
 static std::mutex g_bmu;
@@ -413,6 +449,7 @@ static void free_plan(BatchPlan &p) {
     if (p.graph) (void)hipGraphDestroy(p.graph);
     if (p.d_pairs) (void)hipFree(p.d_pairs);
     if (p.d_sync) (void)hipFree(p.d_sync);
+    if (p.d_z) (void)hipFree(p.d_z);
     p = BatchPlan();
 }
 
@@ -437,8 +474,9 @@ static int launch_batch(const BatchPlan &P, const SetupTaps &st, const Taps &av,
     a.iters = iters;
     a.k = P.k;
     const int nph = (iters + P.k - 1) / P.k;
-    a.phase0 = nph & 1;
     const BVariant &V = kBVariants[P.variant];
+    a.nph = nph;
+    a.phase0 = V.kernel[0] ? nph & 1 : nph == 1;
     if (V.kernel[0]) {
         hipLaunchKernelGGL(V.kernel[av.nd == 8 ? 0 : 1], dim3(P.grid), dim3(V.nt), 0, s, a, st,
                            av);
@@ -537,18 +575,31 @@ extern "C" int hmx_hs_batched_variant_info(int variant, int *tile_w, int *tile_h
 // Device tables of a new plan: the pair descriptors followed by the tile ->
 // pair table, and the barrier / status / claim words.  On failure the caller
 // frees what was allocated (free_plan).
-static int build_plan(BatchPlan &P, const std::vector<PairDesc> &desc, int ntaps, int iters) {
+static int build_plan(BatchPlan &P, std::vector<PairDesc> desc, int ntaps, int iters) {
     const int npairs = (int)desc.size();
     const int tiles = P.total_tiles;
     int rc = resident_grid(P.variant, ntaps, tiles, P.device, P.grid);
     if (rc) return rc;
+    const int64_t nph = (iters + P.k - 1) / P.k;
+    if (!kBVariants[P.variant].kernel[0] && (nph & 1) && nph > 1) {
+        // the stepped form's third plane pair (phase_image_stepped)
+        int64_t total = 0;
+        for (const PairDesc &d : desc) total += 2 * (int64_t)d.H * d.W;
+        HMX_HIP_TRY(hipMalloc(&P.d_z, sizeof(float) * total));
+        int64_t off = 0;
+        for (PairDesc &d : desc) {
+            const int64_t n = (int64_t)d.H * d.W;
+            d.zu = P.d_z + off;
+            d.zv = P.d_z + off + n;
+            off += 2 * n;
+        }
+    }
     std::vector<PairDesc> tile_desc(tiles);
     for (int i = 0; i < npairs; ++i)
         std::fill(tile_desc.begin() + desc[i].tile_base,
                   tile_desc.begin() + desc[i].tile_base + desc[i].ntiles, desc[i]);
     HMX_HIP_TRY(hipMalloc(&P.d_pairs, sizeof(PairDesc) * tiles));
-    const int64_t phases = (iters + P.k - 1) / P.k;
-    P.sync_words = 64 + (phases + 1) * kXcds * kClaimStride;
+    P.sync_words = 64 + (nph + 1) * kXcds * kClaimStride;
     HMX_HIP_TRY(hipMalloc(&P.d_sync, sizeof(unsigned) * P.sync_words));
     HMX_HIP_TRY(hipMemcpy(P.d_pairs, tile_desc.data(), sizeof(PairDesc) * tiles,
                           hipMemcpyHostToDevice));
@@ -628,6 +679,8 @@ static int batched_impl(const hmx_pair *pairs, int npairs, const SetupTaps &st, 
         d.wv = p.work + 4 * n;
         d.u = p.u;
         d.v = p.v;
+        d.zu = p.u;  // (the plan's own planes where the phase count needs them)
+        d.zv = p.v;
         d.alpha2 = p.alpha * p.alpha;
         d.H = p.H;
         d.W = p.W;

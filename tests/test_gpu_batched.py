@@ -52,6 +52,26 @@ def test_all_variants_and_ks():
             check_oracle(pairs, out, 17)
 
 
+def test_phase_counts_one_even_odd():
+    """Every variant at phase counts 1 (the setup's copy), even (two plane
+    pairs) and odd above 1 (the stepped form's third plane pair; the
+    persistent form's copy), with a short last phase, against the oracle."""
+    from hindemith_amd._lib import lib
+    for iters in (1, 8, 13, 16, 24, 35):
+        pairs, B = make(4, seed=12, side=(30, 200), iters=iters)
+        refs = [oracle.hs_solve(p["I1"], p["I2"],
+                                np.float32(p["alpha"]) * np.float32(p["alpha"]), TAPS, iters)
+                for p in pairs]
+        for vi in range(lib().hmx_hs_batched_variant_count()):
+            k = 8 if ctypes_info(vi) >= 8 else 4
+            for graph in (False, True):
+                out = B.run(graph=graph, sync=True, variant=vi, k=k)
+                torch.cuda.synchronize()
+                for (ru, rv), (u, v) in zip(refs, out):
+                    assert np.array_equal(u.cpu().numpy(), ru), (iters, vi, k, graph)
+                    assert np.array_equal(v.cpu().numpy(), rv), (iters, vi, k, graph)
+
+
 def ctypes_info(vi):
     import ctypes
     from hindemith_amd._lib import lib

@@ -13,7 +13,8 @@
 // phases so every tile of every pair sees the previous phase's halo.  Pairs
 // ping-pong between (u, v) and (wu, wv) so that the last phase writes (u, v):
 // with an odd phase count the setup also copies (u, v) into (wu, wv) and the
-// first phase reads from there.
+// first phase reads from there.  (The stepped form copies only for a single
+// phase: it alternates with a third plane pair instead, phase_image_stepped.)
 //
 // The barrier is an arrival counter in device memory: each work-group's
 // first thread releases its writes (agent-scope fence), increments the

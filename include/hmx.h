@@ -350,8 +350,11 @@ struct hmx_pair {
 /* Setup + iters Jacobi iterations of every pair over one list of all pairs'
  * tiles, in phases of k iterations.  Two forms:
  *   stepped (variants 8-11, the default): one launch per phase over every
- *     tile, plus a setup launch (which, for an odd number of phases, also
- *     copies (u, v) into the work planes so the last phase writes (u, v));
+ *     tile, plus a setup launch.  Phases alternate (u, v) with the work
+ *     planes; an odd number of phases above one also uses a plane pair the
+ *     launch state owns (2 x H x W floats per pair), so the last phase writes
+ *     (u, v) and nothing is copied; a single phase reads a copy of (u, v)
+ *     the setup makes in the work planes;
  *   persistent (variants 0-7): ONE launch whose grid, no larger than the GPU
  *     holds at once, walks the list with a grid-wide barrier between phases;
  *     each of the 8 XCDs owns a contiguous share of the list and its
@@ -361,7 +364,8 @@ struct hmx_pair {
  *     stream and returns HMX_ERR_TIMEOUT in that case (so a partial result
  *     is never silent); HMX_BATCH_SYNC makes a stepped call synchronise too.
  * variant = -1 uses the autotuned (or default) variant and k.
- * Launch state (descriptors, barrier words, graph) is cached per signature;
+ * Launch state (descriptors, barrier words, graph, the stepped form's extra
+ * plane pairs) is cached per signature, up to 8 signatures;
  * a graph call on a signature first run eagerly captures the graph then.
  * Calls that overlap on one stream must not share pairs. */
 int hmx_hs_batched(const struct hmx_pair *pairs, int npairs,
